@@ -1,0 +1,315 @@
+// C ABI of the negacyclic product library (include/negacyclic.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/negacyclic.h"
+#include "nc_kernels.hpp"
+#include "nc_plan.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(NC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NC_HIP(call, what)                              \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Default NTT split for one product per call: enough workgroups to spread
+// k * 2^s1 blocks over the chip; large batches are handled by the batch
+// scheduler's own choice.
+uint32_t default_log_blocks(uint32_t log_n) {
+    const uint32_t hi = nc::max_log_blocks(log_n), lo = nc::min_log_blocks(log_n);
+    return std::max(lo, hi);
+}
+
+int grow_workspace(nc_plan* p, uint32_t batch) {
+    if (batch <= p->ws_cap) return NC_OK;
+    NC_HIP(hipDeviceSynchronize(), "workspace sync");
+    if (p->d_res_a) (void)hipFree(p->d_res_a);
+    if (p->d_res_b) (void)hipFree(p->d_res_b);
+    p->d_res_a = p->d_res_b = nullptr;
+    p->ws_cap = 0;
+    const size_t bytes = (size_t)batch * p->host.k * p->host.N * sizeof(uint64_t);
+    NC_HIP(hipMalloc(&p->d_res_a, bytes), "workspace a");
+    NC_HIP(hipMalloc(&p->d_res_b, bytes), "workspace b");
+    p->ws_cap = batch;
+    return NC_OK;
+}
+
+int check_batch(const nc_plan* p, uint32_t batch) {
+    if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
+    if ((uint64_t)batch * p->host.N > (1ull << 31)) return fail(NC_ERR_CAPACITY, "batch * N must not exceed 2^31");
+    return NC_OK;
+}
+
+int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s) {
+    NC_HIP(nc::launch_decompose(p->dev, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s), "decompose launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, p->d_res_a, p->d_res_b, batch, p->log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_crt(p->dev, p->d_res_a, d_c, batch, p->crt_split, s), "crt launch");
+    return NC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nc_status_string(int status) {
+    switch (status) {
+        case NC_OK: return "ok";
+        case NC_ERR_INVALID: return "invalid argument";
+        case NC_ERR_SIZE: return "a and b must contain N*L limbs";
+        case NC_ERR_DEVICE: return "HIP runtime error";
+        case NC_ERR_NOMEM: return "out of memory";
+        case NC_ERR_PRIMES: return "not enough NTT primes for this modulus";
+        case NC_ERR_SELFTEST: return "init self test failed";
+        case NC_ERR_CAPACITY: return "capacity exceeded";
+        case NC_ERR_STATE: return "invalid state";
+        default: return "unknown status";
+    }
+}
+
+const char* nc_last_error(void) { return g_last_error.c_str(); }
+
+void nc_plan_destroy(nc_plan* p) {
+    if (!p) return;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->d_tables) (void)hipFree(p->d_tables);
+    if (p->d_res_a) (void)hipFree(p->d_res_a);
+    if (p->d_res_b) (void)hipFree(p->d_res_b);
+    if (p->d_io) (void)hipFree(p->d_io);
+    if (p->h_pin) (void)hipHostFree(p->h_pin);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
+    if (!out) return fail(NC_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (!q) return fail(NC_ERR_INVALID, "q is null");
+    nc_plan* p = new (std::nothrow) nc_plan();
+    if (!p) return fail(NC_ERR_NOMEM, "plan allocation");
+    try {
+        p->host = nc::build_tables(N, L, q);
+        nc::self_test(p->host);
+    } catch (const std::invalid_argument& e) {
+        delete p;
+        return fail(NC_ERR_INVALID, e.what());
+    } catch (const std::runtime_error& e) {
+        delete p;
+        const std::string m = e.what();
+        return fail(m.rfind("self test", 0) == 0 ? NC_ERR_SELFTEST : NC_ERR_PRIMES, m);
+    } catch (const std::bad_alloc&) {
+        delete p;
+        return fail(NC_ERR_NOMEM, "host tables");
+    }
+    const nc::HostTables& t = p->host;
+    auto cleanup = [&](int rc) {
+        nc_plan_destroy(p);
+        return rc;
+    };
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
+
+    // one resident allocation for all tables
+    const size_t row = (size_t)t.k * t.N;
+    size_t off = 0;
+    const size_t o_twf = off; off = align_up(off + row * sizeof(nc::Tw), 256);
+    const size_t o_twi = off; off = align_up(off + row * sizeof(nc::Tw), 256);
+    const size_t o_pc = off; off = align_up(off + t.k * sizeof(nc::PrimeConst), 256);
+    const size_t o_pl = off; off = align_up(off + t.place.size() * sizeof(uint64_t), 256);
+    const size_t o_cc = off; off = align_up(off + t.k * sizeof(nc::CrtConst), 256);
+    const size_t o_mq = off; off = align_up(off + t.miq.size() * sizeof(uint32_t), 256);
+    const size_t o_qc = off; off = align_up(off + sizeof(nc::QConst), 256);
+    p->table_bytes = off;
+    e = hipMalloc(&p->d_tables, off);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
+    char* base = static_cast<char*>(p->d_tables);
+    struct Up { size_t off; const void* src; size_t bytes; } ups[] = {
+        {o_twf, t.tw_fwd.data(), row * sizeof(nc::Tw)},
+        {o_twi, t.tw_inv.data(), row * sizeof(nc::Tw)},
+        {o_pc, t.pc.data(), t.k * sizeof(nc::PrimeConst)},
+        {o_pl, t.place.data(), t.place.size() * sizeof(uint64_t)},
+        {o_cc, t.cc.data(), t.k * sizeof(nc::CrtConst)},
+        {o_mq, t.miq.data(), t.miq.size() * sizeof(uint32_t)},
+        {o_qc, &t.qc, sizeof(nc::QConst)},
+    };
+    for (const Up& u : ups) {
+        e = hipMemcpy(base + u.off, u.src, u.bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "table upload"));
+    }
+    nc::DevPlan& d = p->dev;
+    d.N = t.N;
+    d.L = t.L;
+    d.k = t.k;
+    d.log_n = t.log_n;
+    d.tw_fwd = reinterpret_cast<const nc::Tw*>(base + o_twf);
+    d.tw_inv = reinterpret_cast<const nc::Tw*>(base + o_twi);
+    d.pc = reinterpret_cast<const nc::PrimeConst*>(base + o_pc);
+    d.place = reinterpret_cast<const uint64_t*>(base + o_pl);
+    d.cc = reinterpret_cast<const nc::CrtConst*>(base + o_cc);
+    d.miq = reinterpret_cast<const uint32_t*>(base + o_mq);
+    d.qc = reinterpret_cast<const nc::QConst*>(base + o_qc);
+
+    e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "stream"));
+    const size_t io = (size_t)t.N * t.L * sizeof(uint32_t);
+    e = hipMalloc(&p->d_io, 3 * io);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "staging"));
+    e = hipHostMalloc(reinterpret_cast<void**>(&p->h_pin), 3 * io, hipHostMallocDefault);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "pinned staging"));
+    int rc = grow_workspace(p, 1);
+    if (rc) return cleanup(rc);
+    p->log_blocks = default_log_blocks(t.log_n);
+    p->crt_split = 1;
+    *out = p;
+    return NC_OK;
+}
+
+int nc_plan_get_info(const nc_plan* p, nc_plan_info* info) {
+    if (!p || !info) return fail(NC_ERR_INVALID, "null argument");
+    info->N = p->host.N;
+    info->L = p->host.L;
+    info->k = p->host.k;
+    info->q_bits = p->host.q_bits;
+    info->log_blocks = p->log_blocks;
+    info->crt_split = p->crt_split;
+    info->table_bytes = p->table_bytes;
+    info->bytes_per_product = 2ull * p->host.k * p->host.N * sizeof(uint64_t);
+    return NC_OK;
+}
+
+int nc_plan_primes(const nc_plan* p, uint64_t* out) {
+    if (!p || !out) return fail(NC_ERR_INVALID, "null argument");
+    std::memcpy(out, p->host.primes.data(), p->host.k * sizeof(uint64_t));
+    return NC_OK;
+}
+
+int nc_plan_set_tuning(nc_plan* p, uint32_t log_blocks, uint32_t crt_split) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (log_blocks != 0xffffffffu) {
+        if (p->host.log_n >= 4 &&
+            (log_blocks < nc::min_log_blocks(p->host.log_n) || log_blocks > nc::max_log_blocks(p->host.log_n)))
+            return fail(NC_ERR_INVALID, "log_blocks out of range for this N");
+        p->log_blocks = p->host.log_n >= 4 ? log_blocks : 0;
+    }
+    if (crt_split != 0xffffffffu) {
+        if (crt_split != 1 && crt_split != 4) return fail(NC_ERR_INVALID, "crt_split must be 1 or 4");
+        p->crt_split = crt_split;
+    }
+    return NC_OK;
+}
+
+int nc_multiply(nc_plan* p, const uint32_t* a, const uint32_t* b, uint32_t* c, size_t count) {
+    if (!p || !a || !b || !c) return fail(NC_ERR_INVALID, "null argument");
+    const size_t n = (size_t)p->host.N * p->host.L;
+    if (count != n) return fail(NC_ERR_SIZE, "a and b must contain N*L limbs");
+    const size_t bytes = n * sizeof(uint32_t);
+    uint32_t* ha = p->h_pin;
+    uint32_t* hb = p->h_pin + n;
+    uint32_t* hc = p->h_pin + 2 * n;
+    uint32_t* da = p->d_io;
+    uint32_t* db = p->d_io + n;
+    uint32_t* dc = p->d_io + 2 * n;
+    std::memcpy(ha, a, bytes);
+    std::memcpy(hb, b, bytes);
+    NC_HIP(hipMemcpyAsync(da, ha, 2 * bytes, hipMemcpyHostToDevice, p->stream), "a/b copy");
+    int rc = run_pipeline(p, da, db, dc, 1, p->stream);
+    if (rc) return rc;
+    NC_HIP(hipMemcpyAsync(hc, dc, bytes, hipMemcpyDeviceToHost, p->stream), "c copy");
+    NC_HIP(hipStreamSynchronize(p->stream), "sync");
+    std::memcpy(c, hc, bytes);
+    return NC_OK;
+}
+
+int nc_multiply_device(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
+                       void* stream) {
+    if (!p || !d_a || !d_b || !d_c) return fail(NC_ERR_INVALID, "null argument");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    rc = grow_workspace(p, batch);
+    if (rc) return rc;
+    return run_pipeline(p, d_a, d_b, d_c, batch, static_cast<hipStream_t>(stream));
+}
+
+int nc_stage_residues(nc_plan* p, const uint32_t* d_poly, uint64_t* d_res, uint32_t batch, void* stream) {
+    if (!p || !d_poly || !d_res) return fail(NC_ERR_INVALID, "null argument");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    NC_HIP(nc::launch_decompose(p->dev, d_poly, d_res, batch, nullptr, nullptr, 0, static_cast<hipStream_t>(stream)),
+           "decompose launch");
+    return NC_OK;
+}
+
+int nc_stage_product_residues(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint64_t* d_res, uint32_t batch,
+                              void* stream) {
+    if (!p || !d_a || !d_b || !d_res) return fail(NC_ERR_INVALID, "null argument");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    rc = grow_workspace(p, batch);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NC_HIP(nc::launch_decompose(p->dev, d_a, d_res, batch, d_b, p->d_res_b, batch, s), "decompose launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, d_res, p->d_res_b, batch, p->log_blocks, s), "ntt launch");
+    return NC_OK;
+}
+
+int nc_stage_crt(nc_plan* p, const uint64_t* d_res, uint32_t* d_c, uint32_t batch, void* stream) {
+    if (!p || !d_res || !d_c) return fail(NC_ERR_INVALID, "null argument");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    NC_HIP(nc::launch_crt(p->dev, d_res, d_c, batch, p->crt_split, static_cast<hipStream_t>(stream)), "crt launch");
+    return NC_OK;
+}
+
+int nc_gen_uniform(uint64_t seed, uint32_t N, uint32_t L, const uint32_t* q, uint32_t* out) {
+    if (!q || !out || L == 0 || L > (uint32_t)nc::kMaxLimbs || N == 0) return fail(NC_ERR_INVALID, "bad argument");
+    int qbits = 0;
+    for (int l = (int)L - 1; l >= 0; --l)
+        if (q[l]) { qbits = l * 32 + 32 - __builtin_clz(q[l]); break; }
+    if (qbits == 0) return fail(NC_ERR_INVALID, "q is zero");
+    const int top = qbits - 32 * ((int)L - 1);
+    const uint32_t mask = top >= 32 ? 0xffffffffu : (top <= 0 ? 0u : ((1u << top) - 1u));
+    const uint32_t words = (L + 1) / 2;
+    uint64_t s = seed;
+    uint32_t cand[nc::kMaxLimbs + 1];
+    for (uint32_t i = 0; i < N; ++i) {
+        for (;;) {
+            for (uint32_t w = 0; w < words; ++w) {
+                uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                cand[2 * w] = (uint32_t)z;
+                cand[2 * w + 1] = (uint32_t)(z >> 32);
+            }
+            cand[L - 1] &= mask;
+            int cmp = 0;
+            for (int l = (int)L - 1; l >= 0 && !cmp; --l)
+                if (cand[l] != q[l]) cmp = cand[l] > q[l] ? 1 : -1;
+            if (cmp < 0) break;
+        }
+        std::memcpy(out + (size_t)i * L, cand, L * sizeof(uint32_t));
+    }
+    return NC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Structures shared by the host plan builder and the gfx950 kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace nc {
+
+constexpr int kMaxLimbs = 32;     // L <= 32 (solve.cu:1170)
+constexpr int kMaxPrimes = 48;    // k <= 48 (solve.cu:22)
+constexpr int kMaxLogN = 16;      // N <= 65536
+constexpr int kMaxLogBlock = 14;  // largest NTT block held in one workgroup's LDS
+constexpr int kMaxOuterBits = 4;  // outer NTT passes cover at most 4 stages
+
+// Per-prime constants for the NTT / pointwise kernels.
+struct PrimeConst {
+    uint64_t p;         // prime, p < 2^62, p = 1 mod 2N
+    uint64_t p2;        // 2p
+    uint64_t mont_neg;  // -p^-1 mod 2^64 (Montgomery REDC constant)
+    uint64_t nr;        // N^-1 * 2^64 mod p: undoes 1/N and the pointwise 2^-64
+    uint64_t nr_s;      // Shoup companion floor(nr 2^64 / p)
+    uint64_t wr;        // psi^-1 * N^-1 * 2^64 mod p: last GS stage twiddle, merged
+    uint64_t wr_s;
+    uint64_t pad;
+};
+
+// Per-prime constants of the CRT (fast CRT, solve.cu:752-836).
+struct CrtConst {
+    uint64_t y;      // (M / p_j)^-1 mod p_j
+    uint64_t y_s;    // Shoup companion
+    uint64_t mag_hi; // floor(2^128 / p_j) = mag_hi 2^64 + mag_lo
+    uint64_t mag_lo;
+};
+
+// Modulus q and its reduction constants, limbs zero padded to kMaxLimbs.
+struct QConst {
+    uint32_t q[kMaxLimbs];
+    uint32_t neg_m[kMaxLimbs];  // (-M) mod q, M = prod p_j
+    uint32_t mu[4];             // floor(2^(bits(q) - 1 + 72) / q), 3 limbs used
+    uint32_t q_bits;
+    uint32_t L;
+    uint32_t pad[2];
+};
+
+// Twiddle pair: value and Shoup companion floor(w 2^64 / p).
+struct Tw {
+    uint64_t w;
+    uint64_t ws;
+};
+
+}  // namespace nc

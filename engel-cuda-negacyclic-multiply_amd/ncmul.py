@@ -1,0 +1,208 @@
+"""Python front end (ctypes) of lib/libnegacyclic.so, the gfx950 library.
+
+Used by the tests, bench.py and __graft_entry__; the product itself is the
+C ABI in include/negacyclic.h.  Device buffers are torch tensors (torch is
+only the allocator and stream provider here).  Every call goes to the HIP
+library; if the library is missing this module raises instead of falling
+back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libnegacyclic.so"
+HEADER = PKG_DIR.parent / "include" / "negacyclic.h"
+
+NC_OK = 0
+STATUS = {
+    0: "NC_OK", -1: "NC_ERR_INVALID", -2: "NC_ERR_SIZE", -3: "NC_ERR_DEVICE", -4: "NC_ERR_NOMEM",
+    -5: "NC_ERR_PRIMES", -6: "NC_ERR_SELFTEST", -7: "NC_ERR_CAPACITY", -8: "NC_ERR_STATE",
+}
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+
+class NcError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {detail}")
+        self.status = status
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_uint32), ("L", ctypes.c_uint32), ("k", ctypes.c_uint32), ("q_bits", ctypes.c_uint32),
+        ("log_blocks", ctypes.c_uint32), ("crt_split", ctypes.c_uint32),
+        ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises FileNotFoundError if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.nc_status_string.restype = ctypes.c_char_p
+        L.nc_status_string.argtypes = [ctypes.c_int]
+        L.nc_last_error.restype = ctypes.c_char_p
+        L.nc_last_error.argtypes = []
+        L.nc_plan_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)]
+        L.nc_plan_destroy.argtypes = [_vp]
+        L.nc_plan_destroy.restype = None
+        L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
+        L.nc_plan_primes.argtypes = [_vp, _u64p]
+        L.nc_plan_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
+        L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_residues.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_product_residues.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_crt.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str):
+    if rc != NC_OK:
+        raise NcError(rc, where, lib().nc_last_error().decode())
+
+
+def _p32(x: np.ndarray):
+    assert x.dtype == np.uint32 and x.flags.c_contiguous
+    return x.ctypes.data_as(_u32p)
+
+
+def gen_uniform(seed: int, N: int, q: np.ndarray) -> np.ndarray:
+    """Seeded uniform polynomial in [0, q) (SplitMix64, see negacyclic.h)."""
+    q = np.ascontiguousarray(q, dtype=np.uint32)
+    out = np.empty((N, q.shape[0]), dtype=np.uint32)
+    _check(lib().nc_gen_uniform(ctypes.c_uint64(seed), N, q.shape[0], _p32(q), _p32(out)), "nc_gen_uniform")
+    return out
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Plan:
+    """One (N, q) plan: resident prime tables on the current device."""
+
+    def __init__(self, N: int, q: np.ndarray):
+        self.q = np.ascontiguousarray(q, dtype=np.uint32)
+        self.N = int(N)
+        self.L = int(self.q.shape[0])
+        h = _vp()
+        _check(lib().nc_plan_create(self.N, self.L, _p32(self.q), ctypes.byref(h)), "nc_plan_create")
+        self._h = h
+        self.info = self.get_info()
+        self.k = self.info.k
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nc_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_info(self) -> PlanInfo:
+        info = PlanInfo()
+        _check(lib().nc_plan_get_info(self._h, ctypes.byref(info)), "nc_plan_get_info")
+        return info
+
+    def primes(self) -> np.ndarray:
+        out = np.empty(self.k, dtype=np.uint64)
+        _check(lib().nc_plan_primes(self._h, out.ctypes.data_as(_u64p)), "nc_plan_primes")
+        return out
+
+    def set_tuning(self, log_blocks=None, crt_split=None):
+        lb = 0xFFFFFFFF if log_blocks is None else int(log_blocks)
+        cs = 0xFFFFFFFF if crt_split is None else int(crt_split)
+        _check(lib().nc_plan_set_tuning(self._h, lb, cs), "nc_plan_set_tuning")
+        self.info = self.get_info()
+
+    # -- host buffers (fherma_run equivalent) --
+    def multiply(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        c = np.empty((self.N, self.L), dtype=np.uint32)
+        _check(lib().nc_multiply(self._h, _p32(a), _p32(b), _p32(c), a.size), "nc_multiply")
+        return c
+
+    # -- device buffers (torch tensors, int32 storage of u32 limbs) --
+    def multiply_device(self, d_a, d_b, d_c, batch: int, stream=None):
+        _check(lib().nc_multiply_device(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()), _vp(d_c.data_ptr()),
+                                        batch, _stream_handle(stream)), "nc_multiply_device")
+
+    def stage_residues(self, d_poly, d_res, batch: int, stream=None):
+        _check(lib().nc_stage_residues(self._h, _vp(d_poly.data_ptr()), _vp(d_res.data_ptr()), batch,
+                                       _stream_handle(stream)), "nc_stage_residues")
+
+    def stage_product_residues(self, d_a, d_b, d_res, batch: int, stream=None):
+        _check(lib().nc_stage_product_residues(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()),
+                                               _vp(d_res.data_ptr()), batch, _stream_handle(stream)),
+               "nc_stage_product_residues")
+
+    def stage_crt(self, d_res, d_c, batch: int, stream=None):
+        _check(lib().nc_stage_crt(self._h, _vp(d_res.data_ptr()), _vp(d_c.data_ptr()), batch,
+                                  _stream_handle(stream)), "nc_stage_crt")
+
+
+# ---------------------------------------------------------------------------
+# numpy <-> torch helpers for u32 / u64 data (torch has no unsigned 32/64-bit
+# arithmetic we need; the bits travel in int32 / int64 storage)
+# ---------------------------------------------------------------------------
+
+def to_device_u32(x: np.ndarray, device="cuda"):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.uint32).view(np.int32)).to(device)
+
+
+def to_device_u64(x: np.ndarray, device="cuda"):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.uint64).view(np.int64)).to(device)
+
+
+def from_device_u32(t) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint32)
+
+
+def from_device_u64(t) -> np.ndarray:
+    return t.cpu().numpy().view(np.uint64)
+
+
+def empty_device_u32(shape, device="cuda"):
+    import torch
+
+    return torch.empty(shape, dtype=torch.int32, device=device)
+
+
+def empty_device_u64(shape, device="cuda"):
+    import torch
+
+    return torch.empty(shape, dtype=torch.int64, device=device)

@@ -1,0 +1,119 @@
+/*
+ * negacyclic.h -- C ABI of the MI355X (gfx950) negacyclic product library.
+ *
+ *   c = a * b  mod (X^N + 1, q)
+ *
+ * a, b, c are N x L arrays of little-endian u32 limbs, coefficient major:
+ * limb l of coefficient i sits at [i*L + l] (the reference layout,
+ * negacyclic.fhk:3-10, main.cpp:25-37, solve.cu:520/835).  Inputs must be
+ * canonical in [0, q); the output is canonical in [0, q).  q is any odd
+ * modulus of L <= 32 limbs with bits(q) + 70 <= 1088 (the reference's caps,
+ * solve.cu:1170 and 1339-1341); N is a power of two in [2, 65536] (the
+ * reference stops at 32768, solve.cu:23).
+ *
+ * The arithmetic is exact: residue-number-system products over k NTT-friendly
+ * primes below 2^62 chosen by the reference's prime rule
+ * (solve.cu:1179-1204), negacyclic NTTs, a pointwise product and a centred
+ * CRT back to L limbs mod q.  Every entry point returns NC_OK (0) or a
+ * negative status; nc_last_error() gives the detail of the last failure on
+ * the calling thread.
+ *
+ * Threading: a plan is not re-entrant (like the reference's State,
+ * solve.cu:28-65); use one plan per host thread, or serialise.
+ */
+#ifndef NEGACYCLIC_H
+#define NEGACYCLIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define NC_OK 0
+#define NC_ERR_INVALID -1    /* bad N, L, q, batch, or a null pointer        */
+#define NC_ERR_SIZE -2       /* a/b/c element count != N*L                   */
+#define NC_ERR_DEVICE -3     /* a HIP runtime error                          */
+#define NC_ERR_NOMEM -4      /* device or host allocation failed             */
+#define NC_ERR_PRIMES -5     /* no prime set covers the coefficient bound    */
+#define NC_ERR_SELFTEST -6   /* an init-time known-answer test failed        */
+#define NC_ERR_CAPACITY -7   /* request exceeds a configured capacity        */
+#define NC_ERR_STATE -8      /* call not valid in the object's current state */
+
+typedef struct nc_plan nc_plan;
+
+/* Plan information (nc_plan_info). */
+typedef struct nc_plan_info {
+    uint32_t N;            /* ring degree                                   */
+    uint32_t L;            /* limbs per coefficient                         */
+    uint32_t k;            /* number of RNS primes                          */
+    uint32_t q_bits;       /* bits(q)                                       */
+    uint32_t log_blocks;   /* NTT outer split: log2(number of LDS blocks)   */
+    uint32_t crt_split;    /* threads per coefficient in the CRT kernel     */
+    uint64_t table_bytes;  /* resident device bytes of the plan's tables    */
+    uint64_t bytes_per_product; /* device workspace bytes per product       */
+} nc_plan_info;
+
+/* Text for a status code. */
+const char* nc_status_string(int status);
+/* Detail of the last failure on this host thread ("" if none). */
+const char* nc_last_error(void);
+
+/* ---- plan: replaces fherma_init / fherma_free ------------------------
+ * nc_plan_create replaces the set-up half of fherma_init
+ * (solve.cu:1150-1487): validation (1167-1172), prime rule (1174-1204),
+ * twiddle / Montgomery / CRT tables (1224-1369, 1396-1398), the
+ * known-answer self tests (1151-1166, 1285-1314, 1370-1415) and the
+ * upload.  nc_plan_destroy replaces fherma_free (solve.cu:1530-1567) and
+ * is null safe. */
+int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out);
+void nc_plan_destroy(nc_plan* plan);
+int nc_plan_get_info(const nc_plan* plan, nc_plan_info* info);
+/* The k primes (host copy), out must hold info.k values. */
+int nc_plan_primes(const nc_plan* plan, uint64_t* out);
+/* Override the NTT outer split (log2 blocks per prime row, 0..4) and the
+ * CRT threads-per-coefficient (1 or 4); 0xffffffff keeps the current value.
+ * Any accepted setting yields bit-identical output. */
+int nc_plan_set_tuning(nc_plan* plan, uint32_t log_blocks, uint32_t crt_split);
+
+/* ---- plain product, host buffers: replaces fherma_run ----------------
+ * solve.cu:1489-1528: a, b in host memory (N*L u32 each), c written to host
+ * memory; count must equal N*L (NC_ERR_SIZE otherwise, like the
+ * reference's runtime_error at 1492-1494).  Returns after c is complete. */
+int nc_multiply(nc_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, size_t count);
+
+/* ---- device-resident batch --------------------------------------------
+ * `batch` independent products; d_a, d_b, d_c are device pointers to
+ * batch x N x L u32 (product-major).  Work is enqueued on `stream` (a
+ * hipStream_t, or NULL for the null stream) and the call returns without
+ * waiting.  Output is bit-identical to nc_multiply on each product. */
+int nc_multiply_device(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
+                       void* stream);
+
+/* ---- stage entry points (device pointers, enqueue on stream) ---------
+ * For stage-level parity checks against the CPU restatement.
+ *  nc_stage_residues:          d_res[p][j][i] = poly_p[i] mod p_j (canonical)
+ *                              (the value solve.cu:533-570 computes before
+ *                              its Montgomery lift and twist)
+ *  nc_stage_product_residues:  d_res[p][j][i] = c_int[i] mod p_j, c_int the
+ *                              exact integer negacyclic product (canonical)
+ *                              (solve.cu:1500-1514 up to untwist)
+ *  nc_stage_crt:               d_c from canonical residues (solve.cu:752-836)
+ * d_res holds batch x k x N u64. */
+int nc_stage_residues(nc_plan* plan, const uint32_t* d_poly, uint64_t* d_res, uint32_t batch, void* stream);
+int nc_stage_product_residues(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint64_t* d_res,
+                              uint32_t batch, void* stream);
+int nc_stage_crt(nc_plan* plan, const uint64_t* d_res, uint32_t* d_c, uint32_t batch, void* stream);
+
+/* ---- synthetic inputs ---------------------------------------------------
+ * Fills out (N x L) with values uniform in [0, q): SplitMix64(seed) stream,
+ * ceil(L/2) u64 words per candidate (word w -> limbs 2w, 2w+1), top limb
+ * masked to bits(q) - 32(L-1) bits, candidates >= q redrawn.  Host only. */
+int nc_gen_uniform(uint64_t seed, uint32_t N, uint32_t L, const uint32_t* q, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEGACYCLIC_H */
