@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Throughput of 868-bit negacyclic products at N = 2^14 on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): products
+c = a*b mod (X^16384 + 1, q), q = 2^868 - 467 (the repo's fixed 868-bit
+prime, L = 28 little-endian u32 limbs), a and b uniform in [0, q) from
+SplitMix64 seeds.  One step = one batch of `--batch` independent products
+through the device-resident batched entry point (nc_multiply_device), inputs
+already in HBM.  Also measured, on the same inputs:
+  * plain: one product per call, device resident, and host-to-host through
+    nc_multiply (the fherma_run replacement; PCIe copies included), with
+    p50 / p99 latency;
+  * per-kernel durations with HIP events on the launch stream, the dominant
+    kernel's roofline against the measured 32x32-bit multiply peak, and its
+    HBM rate;
+  * the CPU baseline: the C restatement of the reference algorithm
+    (oracle/, OpenMP) on this host's cores, on a bounded sample.
+
+Prints one JSON line.  `--gpus N` with N > 1 is refused: this measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "engel-cuda-negacyclic-multiply_amd"
+sys.path.insert(0, str(PKG))
+
+N_DEFAULT = 16384
+Q868 = (1 << 868) - 467
+HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def q_limbs(q: int, L: int) -> np.ndarray:
+    return np.frombuffer(q.to_bytes(4 * L, "little"), dtype="<u4").astype(np.uint32)
+
+
+def work_per_stage(N: int, L: int, k: int, batch: int, s1: int) -> dict:
+    """Algorithmic 32x32-bit multiplies per launch, fixed convention
+    W(N) = N [k (6L + 15 log2 N + 29) + 406] per product, split by stage:
+    decomposition 2L per (coefficient, prime, polynomial); 10 per Shoup
+    butterfly; 11 per pointwise product; CRT 2L + 18 per (coefficient, prime)
+    plus 406 per coefficient for the reduction mod q."""
+    logn = N.bit_length() - 1
+    bfly = 10 * (N // 2) * k * batch
+    return {
+        "decompose": 2 * (2 * L) * N * k * batch,
+        "ntt_outer_fwd": 2 * bfly * s1,
+        "ntt_inner": 3 * bfly * (logn - s1) + 11 * N * k * batch,
+        "ntt_outer_inv": bfly * s1,
+        "crt": (k * (2 * L + 18) + 406) * N * batch,
+    }
+
+
+def bytes_per_stage(N: int, L: int, k: int, batch: int) -> dict:
+    """Compulsory HBM bytes per launch (inputs read + outputs written)."""
+    row = 8 * N * k * batch
+    limbs = 4 * N * L * batch
+    return {
+        "decompose": 2 * limbs + 2 * row,
+        "ntt_outer_fwd": 4 * row,
+        "ntt_inner": 3 * row,
+        "ntt_outer_inv": 2 * row,
+        "crt": row + limbs,
+    }
+
+
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/*pmc*.md), or None when no such summary exists."""
+    for f in sorted((ROOT / "profiles").glob("*pmc*.md")):
+        m = re.search(rf"^traffic_bytes_per_launch\s+{kernel}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+        if m:
+            return float(m.group(1)), f.name
+    return None, None
+
+
+def cpu_baseline(N: int, q: np.ndarray, seconds: float):
+    """The reference algorithm restated in C (oracle/), timed on this host."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    cores = O.cpu_count()
+    plan = O.Plan(N, q)
+    a, b = O.gen_poly(2, N, q), O.gen_poly(3, N, q)
+    plan.multiply(a, b, threads=cores)  # warm
+    done, t0 = 0, time.perf_counter()
+    while True:
+        plan.multiply(a, b, threads=cores)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {
+        "value": done / el, "unit": "products/s", "cores": cores, "kind": "port",
+        "sample": f"{done} products at N={N}, L=28 (868-bit q) in {el:.1f} s: oracle/nc_oracle.c "
+                  f"restatement of solve.cu (prime rule, psi-twisted DIT NTT, Montgomery/Shoup, fast CRT), "
+                  f"OpenMP over primes then coefficients",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=128, help="products per step")
+    ap.add_argument("--N", type=int, default=N_DEFAULT)
+    ap.add_argument("--plain-calls", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: products are independent and this build has no multi-GPU path "
+              "(--gpus must be 1).", file=sys.stderr)
+        return 2
+
+    import torch
+    import ncmul
+
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N, L, B = args.N, 28, args.batch
+    q = q_limbs(Q868, L)
+    stream = torch.cuda.current_stream()
+
+    # inputs: B products, seeds 2i / 2i+1
+    a = np.stack([ncmul.gen_uniform(2 * i, N, q) for i in range(B)])
+    b = np.stack([ncmul.gen_uniform(2 * i + 1, N, q) for i in range(B)])
+    d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+    d_c = ncmul.empty_device_u32((B, N, L))
+
+    batched = ncmul.Plan(N, q)
+    s1_batch = batched.autotune(B)
+    plain = ncmul.Plan(N, q)
+    s1_plain = plain.autotune(1)
+
+    # ---- batched: the timed steps ----
+    for _ in range(args.warmup):
+        batched.multiply_device(d_a, d_b, d_c, B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched.multiply_device(d_a, d_b, d_c, B)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    batched_pps = B * args.steps / elapsed
+    c_batch = ncmul.from_device_u32(d_c)
+
+    # ---- plain, device resident: one product per call ----
+    for _ in range(5):
+        plain.multiply_device(d_a[0], d_b[0], d_c[0], 1)
+    torch.cuda.synchronize()
+    calls = args.plain_calls
+    t0 = time.perf_counter()
+    for i in range(calls):
+        plain.multiply_device(d_a[i % B], d_b[i % B], d_c[i % B], 1)
+    torch.cuda.synchronize()
+    plain_dev_pps = calls / (time.perf_counter() - t0)
+
+    # ---- plain, host to host (fherma_run convention), latency ----
+    lat = []
+    for i in range(calls):
+        t = time.perf_counter()
+        c_host = plain.multiply(a[i % B], b[i % B])
+        lat.append((time.perf_counter() - t) * 1e3)
+        if i < 4:
+            assert np.array_equal(c_host, c_batch[i % B]), "plain and batched differ"
+    lat.sort()
+    p50 = statistics.median(lat)
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+
+    # bit-exactness of the batched step against the CPU restatement (2 products)
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    oplan = O.Plan(N, q)
+    bit_exact = all(np.array_equal(c_batch[i], oplan.multiply(a[i], b[i])) for i in (0, B - 1))
+
+    # ---- per-kernel durations (HIP events on the launch stream) ----
+    stage_ms = batched.profile_stages(d_a, d_b, d_c, B, reps=5, stream=stream)
+    k = batched.k
+    work = work_per_stage(N, L, k, B, s1_batch)
+    byts = bytes_per_stage(N, L, k, B)
+    peak_mul = ncmul.measure_mul32_peak()
+    dom = max(stage_ms, key=lambda s: stage_ms[s])
+    dom_s = stage_ms[dom] * 1e-3
+    traffic, traffic_src = pmc_traffic(dom)
+    roofline = {
+        "bound": "valu",
+        "kernel": dom,
+        "achieved": work[dom] / dom_s,
+        "peak": peak_mul,
+        "unit": "mul32/s",
+        "frac": work[dom] / dom_s / peak_mul,
+        "traffic": traffic,
+        "peak_source": "nc_measure_mul32_peak (v_mad_u64_u32 chains on every CU, this run)",
+        "work_per_launch": work[dom],
+        "traffic_source": traffic_src,
+    }
+    roofline_hbm = {
+        "bound": "hbm", "kernel": dom, "achieved": byts[dom] / dom_s, "peak": HBM_PEAK, "unit": "B/s",
+        "frac": byts[dom] / dom_s / HBM_PEAK, "bytes_per_launch": byts[dom],
+    }
+    W = N * (k * (6 * L + 15 * (N.bit_length() - 1) + 29) + 406)
+    step_total_ms = sum(stage_ms.values())
+
+    out = {
+        "metric": "868-bit negacyclic products/sec at N=2^14 (batched vs plain, bit-exact); p99 ms",
+        "value": batched_pps,
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (SplitMix64 seeds, uniform in [0, q))",
+        "config": {
+            "workload": f"negacyclic product mod (X^{N}+1, q), q = 2^868-467 (L=28 u32 limbs), "
+                        f"{B} products per step, device resident",
+            "N": N, "L": L, "k_primes": k, "batch": B, "ntt_split_batched": s1_batch, "ntt_split_plain": s1_plain,
+        },
+        "plain": {
+            "device_products_per_s": plain_dev_pps,
+            "host_to_host_products_per_s": 1e3 / statistics.mean(lat),
+            "p50_ms": p50, "p99_ms": p99, "calls": calls,
+        },
+        "batched_over_plain": batched_pps / plain_dev_pps,
+        "p99_ms": p99,
+        "bit_exact": bit_exact,
+        "stage_ms": stage_ms,
+        "roofline": roofline,
+        "roofline_hbm": roofline_hbm,
+        "w_convention": {
+            "W_per_product": W,
+            "whole_path_mul32_per_s": batched_pps * W,
+            "frac_of_peak": batched_pps * W / peak_mul,
+            "profiled_step_ms": step_total_ms,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, q, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0 if bit_exact else 3
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -33,12 +33,31 @@ int hip_fail(hipError_t e, const char* what) {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Default NTT split for one product per call: enough workgroups to spread
-// k * 2^s1 blocks over the chip; large batches are handled by the batch
-// scheduler's own choice.
-uint32_t default_log_blocks(uint32_t log_n) {
-    const uint32_t hi = nc::max_log_blocks(log_n), lo = nc::min_log_blocks(log_n);
-    return std::max(lo, hi);
+uint32_t bucket_of(uint32_t batch) {
+    uint32_t b = 0;
+    while ((1u << b) < batch && b < 31) ++b;
+    return b;
+}
+
+// Untuned default: the smallest outer split that still gives about 1024
+// workgroup-sized blocks (k * batch * 2^s1), since fewer, larger blocks cost
+// fewer HBM round trips but must still fill 256 compute units.
+uint32_t heuristic_log_blocks(uint32_t log_n, uint32_t k, uint32_t batch) {
+    const uint32_t lo = nc::min_log_blocks(log_n), hi = std::max(lo, nc::max_log_blocks(log_n));
+    uint32_t s = lo;
+    while (s < hi && (uint64_t)k * batch * (1ull << s) < 1024) ++s;
+    return s;
+}
+
+struct Choice {
+    uint32_t log_blocks, crt_split;
+};
+
+Choice choose(const nc_plan* p, uint32_t batch) {
+    if (p->forced || p->host.log_n < 4) return {p->host.log_n < 4 ? 0u : p->log_blocks, p->crt_split};
+    const nc_plan::Tuning& t = p->tune[bucket_of(batch)];
+    if (t.tuned) return {t.log_blocks, t.crt_split};
+    return {heuristic_log_blocks(p->host.log_n, p->host.k, batch), 1u};
 }
 
 int grow_workspace(nc_plan* p, uint32_t batch) {
@@ -61,10 +80,12 @@ int check_batch(const nc_plan* p, uint32_t batch) {
     return NC_OK;
 }
 
-int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s) {
+int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
+                 const Choice* force = nullptr) {
+    const Choice c = force ? *force : choose(p, batch);
     NC_HIP(nc::launch_decompose(p->dev, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s), "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, p->d_res_a, p->d_res_b, batch, p->log_blocks, s), "ntt launch");
-    NC_HIP(nc::launch_crt(p->dev, p->d_res_a, d_c, batch, p->crt_split, s), "crt launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_crt(p->dev, p->d_res_a, d_c, batch, c.crt_split, s), "crt launch");
     return NC_OK;
 }
 
@@ -178,7 +199,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     if (e != hipSuccess) return cleanup(hip_fail(e, "pinned staging"));
     int rc = grow_workspace(p, 1);
     if (rc) return cleanup(rc);
-    p->log_blocks = default_log_blocks(t.log_n);
+    p->log_blocks = heuristic_log_blocks(t.log_n, t.k, 1);
     p->crt_split = 1;
     *out = p;
     return NC_OK;
@@ -190,8 +211,9 @@ int nc_plan_get_info(const nc_plan* p, nc_plan_info* info) {
     info->L = p->host.L;
     info->k = p->host.k;
     info->q_bits = p->host.q_bits;
-    info->log_blocks = p->log_blocks;
-    info->crt_split = p->crt_split;
+    const Choice c = choose(p, 1);
+    info->log_blocks = c.log_blocks;
+    info->crt_split = c.crt_split;
     info->table_bytes = p->table_bytes;
     info->bytes_per_product = 2ull * p->host.k * p->host.N * sizeof(uint64_t);
     return NC_OK;
@@ -210,11 +232,66 @@ int nc_plan_set_tuning(nc_plan* p, uint32_t log_blocks, uint32_t crt_split) {
             (log_blocks < nc::min_log_blocks(p->host.log_n) || log_blocks > nc::max_log_blocks(p->host.log_n)))
             return fail(NC_ERR_INVALID, "log_blocks out of range for this N");
         p->log_blocks = p->host.log_n >= 4 ? log_blocks : 0;
+        p->forced = true;
     }
     if (crt_split != 0xffffffffu) {
         if (crt_split != 1 && crt_split != 4) return fail(NC_ERR_INVALID, "crt_split must be 1 or 4");
         p->crt_split = crt_split;
+        p->forced = true;
     }
+    return NC_OK;
+}
+
+int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_log_blocks) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    rc = grow_workspace(p, batch);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t n = (size_t)batch * p->host.N * p->host.L;
+    uint32_t* buf = nullptr;
+    NC_HIP(hipMalloc(&buf, 3 * n * sizeof(uint32_t)), "autotune buffers");
+    // zero inputs are valid (0 < q) and time exactly like any other input:
+    // every kernel's work is data independent
+    hipError_t e = hipMemsetAsync(buf, 0, 2 * n * sizeof(uint32_t), s);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    Choice best{heuristic_log_blocks(p->host.log_n, p->host.k, batch), 1u};
+    float best_ms = -1.f;
+    if (p->host.log_n >= 4) {
+        const uint32_t lo = nc::min_log_blocks(p->host.log_n), hi = std::max(lo, nc::max_log_blocks(p->host.log_n));
+        for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb) {
+            const Choice c{lb, 1u};
+            float ms = 0.f;
+            for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {  // rep 0 warms up
+                e = hipEventRecord(t0, s);
+                if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, &c) != NC_OK)
+                    e = hipErrorLaunchFailure;
+                if (e == hipSuccess) e = hipEventRecord(t1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(t1);
+                float t = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+                if (rep > 0) ms += t;
+            }
+            if (e == hipSuccess && (best_ms < 0.f || ms < best_ms)) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "autotune");
+    nc_plan::Tuning& t = p->tune[bucket_of(batch)];
+    t.log_blocks = (uint8_t)best.log_blocks;
+    t.crt_split = (uint8_t)best.crt_split;
+    t.tuned = 1;
+    p->forced = false;
+    if (chosen_log_blocks) *chosen_log_blocks = best.log_blocks;
     return NC_OK;
 }
 
@@ -268,7 +345,7 @@ int nc_stage_product_residues(nc_plan* p, const uint32_t* d_a, const uint32_t* d
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     NC_HIP(nc::launch_decompose(p->dev, d_a, d_res, batch, d_b, p->d_res_b, batch, s), "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, d_res, p->d_res_b, batch, p->log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, d_res, p->d_res_b, batch, choose(p, batch).log_blocks, s), "ntt launch");
     return NC_OK;
 }
 
@@ -276,7 +353,85 @@ int nc_stage_crt(nc_plan* p, const uint64_t* d_res, uint32_t* d_c, uint32_t batc
     if (!p || !d_res || !d_c) return fail(NC_ERR_INVALID, "null argument");
     int rc = check_batch(p, batch);
     if (rc) return rc;
-    NC_HIP(nc::launch_crt(p->dev, d_res, d_c, batch, p->crt_split, static_cast<hipStream_t>(stream)), "crt launch");
+    NC_HIP(nc::launch_crt(p->dev, d_res, d_c, batch, choose(p, batch).crt_split, static_cast<hipStream_t>(stream)),
+           "crt launch");
+    return NC_OK;
+}
+
+int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
+                      uint32_t reps, void* stream, float* ms) {
+    if (!p || !d_a || !d_b || !d_c || !ms || reps == 0) return fail(NC_ERR_INVALID, "bad argument");
+    int rc = check_batch(p, batch);
+    if (rc) return rc;
+    rc = grow_workspace(p, batch);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Choice c = choose(p, batch);
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; ++i) NC_HIP(hipEventCreate(&ev[i]), "event");
+    double acc[5] = {0, 0, 0, 0, 0};
+    int status = NC_OK;
+    for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
+        hipError_t e = hipEventRecord(ev[0], s);
+        if (e == hipSuccess) e = nc::launch_decompose(p->dev, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+        if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[3], s);
+        if (e == hipSuccess) e = nc::launch_ntt_outer_inv(p->dev, p->d_res_a, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+        if (e == hipSuccess) e = nc::launch_crt(p->dev, p->d_res_a, d_c, batch, c.crt_split, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[5], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
+        for (int i = 0; i < 5 && e == hipSuccess; ++i) {
+            float t = 0;
+            e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+            acc[i] += t;
+        }
+        if (e != hipSuccess) status = hip_fail(e, "profile");
+    }
+    for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
+    if (status) return status;
+    const bool outer = p->host.log_n >= 4 && c.log_blocks > 0;
+    for (int i = 0; i < 5; ++i) ms[i] = (float)(acc[i] / reps);
+    if (!outer) ms[1] = ms[3] = 0.f;
+    return NC_OK;
+}
+
+int nc_measure_mul32_peak(double* out) {
+    if (!out) return fail(NC_ERR_INVALID, "null argument");
+    hipDeviceProp_t prop;
+    NC_HIP(hipGetDeviceProperties(&prop, 0), "device properties");
+    const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16;  // 16 waves of 64 lanes per CU
+    const uint32_t iters = 4096;
+    uint32_t* d_in = nullptr;
+    uint64_t* d_out = nullptr;
+    NC_HIP(hipMalloc(&d_in, 64 * sizeof(uint32_t)), "probe input");
+    hipError_t e = hipMalloc(&d_out, (size_t)blocks * 256 * sizeof(uint64_t));
+    uint32_t h_in[64];
+    for (int i = 0; i < 64; ++i) h_in[i] = 0x9E3779B9u * (uint32_t)(i + 1);
+    if (e == hipSuccess) e = hipMemcpy(d_in, h_in, sizeof h_in, hipMemcpyHostToDevice);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    float best = 0;
+    for (int rep = 0; rep < 4 && e == hipSuccess; ++rep) {  // first launch warms up
+        e = hipEventRecord(t0, nullptr);
+        if (e == hipSuccess) e = nc::launch_mul32_peak(d_in, d_out, blocks, iters, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (rep > 0 && (best == 0 || ms < best)) best = ms;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) return hip_fail(e, "mul32 probe");
+    const double muls = (double)blocks * 256.0 * iters * 8.0;
+    *out = muls / (best * 1e-3);
     return NC_OK;
 }
 

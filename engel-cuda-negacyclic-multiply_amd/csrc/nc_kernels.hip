@@ -588,14 +588,9 @@ static void launch_outer_inv(const DevPlan& d, uint64_t* ra, uint32_t nprod, hip
     ntt_outer_inv_kernel<S1><<<grid, 256, 0, s>>>(ra, d.tw_inv, d.pc, d.log_n, d.k, nprod);
 }
 
-hipError_t launch_ntt_product(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
-                              hipStream_t s) {
-    if (d.log_n < 4) {
-        const uint32_t t = d.k * nprod;
-        ntt_tiny_kernel<<<(t + 63) / 64, 64, 0, s>>>(ra, rb, d.tw_fwd, d.tw_inv, d.pc, d.log_n, d.k, nprod);
-        return hipGetLastError();
-    }
-    if (log_blocks < min_log_blocks(d.log_n) || log_blocks > max_log_blocks(d.log_n)) return hipErrorInvalidValue;
+hipError_t launch_ntt_outer_fwd(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
+                                hipStream_t s) {
+    if (d.log_n < 4) return hipSuccess;
     switch (log_blocks) {
         case 0: break;
         case 1: launch_outer_fwd<1>(d, ra, rb, nprod, s); break;
@@ -603,6 +598,16 @@ hipError_t launch_ntt_product(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint
         case 3: launch_outer_fwd<3>(d, ra, rb, nprod, s); break;
         case 4: launch_outer_fwd<4>(d, ra, rb, nprod, s); break;
         default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt_inner(const DevPlan& d, uint64_t* ra, const uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
+                            hipStream_t s) {
+    if (d.log_n < 4) {
+        const uint32_t t = d.k * nprod;
+        ntt_tiny_kernel<<<(t + 63) / 64, 64, 0, s>>>(ra, rb, d.tw_fwd, d.tw_inv, d.pc, d.log_n, d.k, nprod);
+        return hipGetLastError();
     }
     switch (d.log_n - log_blocks) {
         case 4: launch_inner<4>(d, ra, rb, nprod, s); break;
@@ -618,14 +623,31 @@ hipError_t launch_ntt_product(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint
         case 14: launch_inner<14>(d, ra, rb, nprod, s); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt_outer_inv(const DevPlan& d, uint64_t* ra, uint32_t nprod, uint32_t log_blocks, hipStream_t s) {
+    if (d.log_n < 4) return hipSuccess;
     switch (log_blocks) {
         case 0: break;
         case 1: launch_outer_inv<1>(d, ra, nprod, s); break;
         case 2: launch_outer_inv<2>(d, ra, nprod, s); break;
         case 3: launch_outer_inv<3>(d, ra, nprod, s); break;
         case 4: launch_outer_inv<4>(d, ra, nprod, s); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_ntt_product(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
+                              hipStream_t s) {
+    if (d.log_n >= 4 && (log_blocks < min_log_blocks(d.log_n) || log_blocks > max_log_blocks(d.log_n)))
+        return hipErrorInvalidValue;
+    hipError_t e = launch_ntt_outer_fwd(d, ra, rb, nprod, log_blocks, s);
+    if (e != hipSuccess) return e;
+    e = launch_ntt_inner(d, ra, rb, nprod, log_blocks, s);
+    if (e != hipSuccess) return e;
+    return launch_ntt_outer_inv(d, ra, nprod, log_blocks, s);
 }
 
 hipError_t launch_crt(const DevPlan& d, const uint64_t* res, uint32_t* out, uint32_t nprod, uint32_t split,
@@ -637,6 +659,33 @@ hipError_t launch_crt(const DevPlan& d, const uint64_t* res, uint32_t* out, uint
         crt_kernel<28><<<grid, 256, 0, s>>>(res, out, d.cc, d.miq, d.qc, d.pc, d.log_n, d.L, d.k, nprod);
     else
         crt_kernel<kMaxLimbs><<<grid, 256, 0, s>>>(res, out, d.cc, d.miq, d.qc, d.pc, d.log_n, d.L, d.k, nprod);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 32x32->64 multiply throughput probe: eight independent v_mad_u64_u32 chains
+// per lane, no other VALU work in the loop.  Used to measure the integer
+// multiply peak that bounds this library's kernels.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void mul32_peak_kernel(const uint32_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                         uint32_t iters) {
+    const uint32_t b = in[threadIdx.x & 63u] | 1u;
+    const uint64_t c = in[(threadIdx.x + 7u) & 63u];
+    uint64_t acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = in[(threadIdx.x + (uint32_t)u) & 63u];
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = (uint64_t)(uint32_t)acc[u] * b + c;
+    }
+    uint64_t x = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x ^= acc[u];
+    out[blockIdx.x * 256 + threadIdx.x] = x;
+}
+
+hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
+    mul32_peak_kernel<<<blocks, 256, 0, s>>>(in, out, iters);
     return hipGetLastError();
 }
 

@@ -32,6 +32,16 @@ hipError_t launch_decompose(const DevPlan& d, const uint32_t* a, uint64_t* res_a
 hipError_t launch_ntt_product(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
                               hipStream_t s);
 
+// The three parts of launch_ntt_product, for per-stage timing.
+hipError_t launch_ntt_outer_fwd(const DevPlan& d, uint64_t* ra, uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
+                                hipStream_t s);
+hipError_t launch_ntt_inner(const DevPlan& d, uint64_t* ra, const uint64_t* rb, uint32_t nprod, uint32_t log_blocks,
+                            hipStream_t s);
+hipError_t launch_ntt_outer_inv(const DevPlan& d, uint64_t* ra, uint32_t nprod, uint32_t log_blocks, hipStream_t s);
+
+// 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
+hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
+
 // canonical residues -> L limbs mod q. split = threads per coefficient (1 or 4).
 hipError_t launch_crt(const DevPlan& d, const uint64_t* res, uint32_t* out, uint32_t nprod, uint32_t split,
                       hipStream_t s);

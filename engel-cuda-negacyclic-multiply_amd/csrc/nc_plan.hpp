@@ -49,6 +49,16 @@ struct nc_plan {
     uint32_t* d_io = nullptr;     // a | b | c, N*L u32 each
     uint32_t* h_pin = nullptr;    // pinned a | b | c
     hipStream_t stream = nullptr;
-    uint32_t log_blocks = 0;      // NTT outer split
+    // Tuning per batch-size bucket (bucket = ceil(log2 batch), 0..31): NTT
+    // outer split and CRT threads per coefficient.  `forced` pins every
+    // bucket to log_blocks / crt_split (nc_plan_set_tuning).
+    struct Tuning {
+        uint8_t log_blocks = 0;
+        uint8_t crt_split = 1;
+        uint8_t tuned = 0;
+    };
+    Tuning tune[32];
+    bool forced = false;
+    uint32_t log_blocks = 0;      // forced / reported NTT outer split
     uint32_t crt_split = 1;
 };

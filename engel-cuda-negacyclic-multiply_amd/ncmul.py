@@ -68,6 +68,10 @@ def lib():
         L.nc_stage_product_residues.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_crt.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
+        L.nc_plan_autotune.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.nc_profile_stages.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
+                                        ctypes.POINTER(ctypes.c_float)]
+        L.nc_measure_mul32_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -88,6 +92,13 @@ def gen_uniform(seed: int, N: int, q: np.ndarray) -> np.ndarray:
     out = np.empty((N, q.shape[0]), dtype=np.uint32)
     _check(lib().nc_gen_uniform(ctypes.c_uint64(seed), N, q.shape[0], _p32(q), _p32(out)), "nc_gen_uniform")
     return out
+
+
+def measure_mul32_peak() -> float:
+    """Measured 32x32->64 integer multiplies per second of the current device."""
+    out = ctypes.c_double()
+    _check(lib().nc_measure_mul32_peak(ctypes.byref(out)), "nc_measure_mul32_peak")
+    return out.value
 
 
 def _stream_handle(stream):
@@ -143,6 +154,19 @@ class Plan:
         cs = 0xFFFFFFFF if crt_split is None else int(crt_split)
         _check(lib().nc_plan_set_tuning(self._h, lb, cs), "nc_plan_set_tuning")
         self.info = self.get_info()
+
+    def autotune(self, batch: int, stream=None) -> int:
+        chosen = ctypes.c_uint32()
+        _check(lib().nc_plan_autotune(self._h, batch, _stream_handle(stream), ctypes.byref(chosen)),
+               "nc_plan_autotune")
+        return int(chosen.value)
+
+    def profile_stages(self, d_a, d_b, d_c, batch: int, reps: int = 5, stream=None):
+        """Mean per-launch milliseconds: decompose, ntt_outer_fwd, ntt_inner, ntt_outer_inv, crt."""
+        ms = (ctypes.c_float * 5)()
+        _check(lib().nc_profile_stages(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()), _vp(d_c.data_ptr()),
+                                       batch, reps, _stream_handle(stream), ms), "nc_profile_stages")
+        return dict(zip(("decompose", "ntt_outer_fwd", "ntt_inner", "ntt_outer_inv", "crt"), list(ms)))
 
     # -- host buffers (fherma_run equivalent) --
     def multiply(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:

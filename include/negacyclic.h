@@ -78,6 +78,13 @@ int nc_plan_primes(const nc_plan* plan, uint64_t* out);
  * Any accepted setting yields bit-identical output. */
 int nc_plan_set_tuning(nc_plan* plan, uint32_t log_blocks, uint32_t crt_split);
 
+/* Autotuner: times every NTT split for `batch` products on `stream` (zero
+ * inputs; the kernels' work does not depend on the data) and keeps the
+ * fastest for batches of that size bucket (ceil(log2 batch)).  Clears a
+ * setting forced by nc_plan_set_tuning.  Output bits never depend on it.
+ * (The reference has a single fixed schedule, solve.cu:1006-1038.) */
+int nc_plan_autotune(nc_plan* plan, uint32_t batch, void* stream, uint32_t* chosen_log_blocks);
+
 /* ---- plain product, host buffers: replaces fherma_run ----------------
  * solve.cu:1489-1528: a, b in host memory (N*L u32 each), c written to host
  * memory; count must equal N*L (NC_ERR_SIZE otherwise, like the
@@ -106,6 +113,22 @@ int nc_stage_residues(nc_plan* plan, const uint32_t* d_poly, uint64_t* d_res, ui
 int nc_stage_product_residues(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint64_t* d_res,
                               uint32_t batch, void* stream);
 int nc_stage_crt(nc_plan* plan, const uint64_t* d_res, uint32_t* d_c, uint32_t batch, void* stream);
+
+/* ---- diagnostics -------------------------------------------------------
+ * nc_profile_stages runs the batched pipeline `reps` times on `stream` and
+ * returns the mean duration in milliseconds of each kernel launch, measured
+ * with HIP events recorded on that stream between the launches:
+ *   ms[0] decompose, ms[1] NTT outer forward, ms[2] NTT inner (forward,
+ *   pointwise, inverse), ms[3] NTT outer inverse, ms[4] CRT.
+ * A stage the plan's split does not launch reports 0.  Blocks until done.
+ * (No counterpart in the reference, which has no timing or tracing.) */
+int nc_profile_stages(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
+                      uint32_t reps, void* stream, float* ms);
+/* Measured 32x32->64-bit integer multiply throughput of the current device
+ * (v_mad_u64_u32 chains, all compute units busy), in multiplies per second.
+ * The integer multiplier is what bounds this library's kernels; this is the
+ * peak their roofline fraction is quoted against.  Blocks until done. */
+int nc_measure_mul32_peak(double* mul32_per_second);
 
 /* ---- synthetic inputs ---------------------------------------------------
  * Fills out (N x L) with values uniform in [0, q): SplitMix64(seed) stream,
