@@ -12,6 +12,8 @@
 #include "nc_kernels.hpp"
 #include "nc_plan.hpp"
 
+using nc::Choice;
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -25,9 +27,9 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(NC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define NC_HIP(call, what)                              \
-    do {                                                \
-        hipError_t e_ = (call);                         \
+#define NC_HIP(call, what)                               \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
@@ -39,25 +41,30 @@ uint32_t bucket_of(uint32_t batch) {
     return b;
 }
 
-// Untuned default: the smallest outer split that still gives about 1024
-// workgroup-sized blocks (k * batch * 2^s1), since fewer, larger blocks cost
-// fewer HBM round trips but must still fill 256 compute units.
-uint32_t heuristic_log_blocks(uint32_t log_n, uint32_t k, uint32_t batch) {
-    const uint32_t lo = nc::min_log_blocks(log_n), hi = std::max(lo, nc::max_log_blocks(log_n));
-    uint32_t s = lo;
-    while (s < hi && (uint64_t)k * batch * (1ull << s) < 1024) ++s;
-    return s;
+uint32_t k_of(const nc_plan* p, uint32_t width) {
+    return width == 62 ? (uint32_t)p->host.s62.primes.size() : (uint32_t)p->host.s30.primes.size();
 }
 
-struct Choice {
-    uint32_t log_blocks, crt_split;
-};
+// Untuned default: 30-bit primes, and the smallest outer split that still
+// gives about 1024 workgroup-sized blocks (k * batch * 2^s1): fewer, larger
+// blocks cost fewer HBM round trips but must still fill 256 compute units.
+Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
+    const uint32_t log_n = p->host.log_n;
+    Choice c;
+    c.width = width;
+    if (log_n < 4) return c;
+    const uint32_t lo = nc::min_log_blocks(log_n), hi = std::max(lo, nc::max_log_blocks(log_n));
+    uint32_t s = lo;
+    while (s < hi && (uint64_t)k_of(p, width) * batch * (1ull << s) < 1024) ++s;
+    c.log_blocks = s;
+    return c;
+}
 
 Choice choose(const nc_plan* p, uint32_t batch) {
-    if (p->forced || p->host.log_n < 4) return {p->host.log_n < 4 ? 0u : p->log_blocks, p->crt_split};
-    const nc_plan::Tuning& t = p->tune[bucket_of(batch)];
-    if (t.tuned) return {t.log_blocks, t.crt_split};
-    return {heuristic_log_blocks(p->host.log_n, p->host.k, batch), 1u};
+    if (p->forced) return p->forced_choice;
+    const nc_plan::Tuned& t = p->tune[bucket_of(batch)];
+    if (t.tuned) return t.choice;
+    return heuristic(p, batch);
 }
 
 int grow_workspace(nc_plan* p, uint32_t batch) {
@@ -67,7 +74,7 @@ int grow_workspace(nc_plan* p, uint32_t batch) {
     if (p->d_res_b) (void)hipFree(p->d_res_b);
     p->d_res_a = p->d_res_b = nullptr;
     p->ws_cap = 0;
-    const size_t bytes = (size_t)batch * p->host.k * p->host.N * sizeof(uint64_t);
+    const size_t bytes = (size_t)batch * p->res_bytes;
     NC_HIP(hipMalloc(&p->d_res_a, bytes), "workspace a");
     NC_HIP(hipMalloc(&p->d_res_b, bytes), "workspace b");
     p->ws_cap = batch;
@@ -80,13 +87,24 @@ int check_batch(const nc_plan* p, uint32_t batch) {
     return NC_OK;
 }
 
-int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
-                 const Choice* force = nullptr) {
-    const Choice c = force ? *force : choose(p, batch);
-    NC_HIP(nc::launch_decompose(p->dev, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s), "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
-    NC_HIP(nc::launch_crt(p->dev, p->d_res_a, d_c, batch, c.crt_split, s), "crt launch");
+int check_width(uint32_t width) {
+    if (width != 62 && width != 30) return fail(NC_ERR_INVALID, "prime width must be 62 or 30");
     return NC_OK;
+}
+
+int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
+                 const Choice& c) {
+    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s),
+           "decompose launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s), "crt launch");
+    return NC_OK;
+}
+
+// Upload one host table vector at `off` of the table allocation.
+template <class V>
+hipError_t upload(char* base, size_t off, const V& v) {
+    return hipMemcpy(base + off, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
 }
 
 }  // namespace
@@ -150,45 +168,58 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
 
-    // one resident allocation for all tables
-    const size_t row = (size_t)t.k * t.N;
+    // one resident allocation for both prime sets' tables
+    const nc::HostSet62& a = t.s62;
+    const nc::HostSet30& b = t.s30;
     size_t off = 0;
-    const size_t o_twf = off; off = align_up(off + row * sizeof(nc::Tw), 256);
-    const size_t o_twi = off; off = align_up(off + row * sizeof(nc::Tw), 256);
-    const size_t o_pc = off; off = align_up(off + t.k * sizeof(nc::PrimeConst), 256);
-    const size_t o_pl = off; off = align_up(off + t.place.size() * sizeof(uint64_t), 256);
-    const size_t o_cc = off; off = align_up(off + t.k * sizeof(nc::CrtConst), 256);
-    const size_t o_mq = off; off = align_up(off + t.miq.size() * sizeof(uint32_t), 256);
-    const size_t o_qc = off; off = align_up(off + sizeof(nc::QConst), 256);
+    auto slot = [&](size_t bytes) {
+        const size_t at = off;
+        off = align_up(off + bytes, 256);
+        return at;
+    };
+    const size_t o62_twf = slot(a.tw_fwd.size() * sizeof(nc::Tw)), o62_twi = slot(a.tw_inv.size() * sizeof(nc::Tw));
+    const size_t o62_pc = slot(a.pc.size() * sizeof(nc::PrimeConst)), o62_pl = slot(a.place.size() * 8);
+    const size_t o62_cc = slot(a.cc.size() * sizeof(nc::CrtConst)), o62_mq = slot(a.miq.size() * 4);
+    const size_t o62_qc = slot(sizeof(nc::QConst));
+    const size_t o30_twf = slot(b.tw_fwd.size() * sizeof(nc::Tw32)), o30_twi = slot(b.tw_inv.size() * sizeof(nc::Tw32));
+    const size_t o30_pc = slot(b.pc.size() * sizeof(nc::PrimeConst32)), o30_pl = slot(b.place.size() * 4);
+    const size_t o30_cc = slot(b.cc.size() * sizeof(nc::CrtConst32)), o30_mq = slot(b.miq.size() * 4);
+    const size_t o30_qc = slot(sizeof(nc::QConst));
     p->table_bytes = off;
     e = hipMalloc(&p->d_tables, off);
     if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
     char* base = static_cast<char*>(p->d_tables);
-    struct Up { size_t off; const void* src; size_t bytes; } ups[] = {
-        {o_twf, t.tw_fwd.data(), row * sizeof(nc::Tw)},
-        {o_twi, t.tw_inv.data(), row * sizeof(nc::Tw)},
-        {o_pc, t.pc.data(), t.k * sizeof(nc::PrimeConst)},
-        {o_pl, t.place.data(), t.place.size() * sizeof(uint64_t)},
-        {o_cc, t.cc.data(), t.k * sizeof(nc::CrtConst)},
-        {o_mq, t.miq.data(), t.miq.size() * sizeof(uint32_t)},
-        {o_qc, &t.qc, sizeof(nc::QConst)},
+    const hipError_t ups[] = {
+        upload(base, o62_twf, a.tw_fwd), upload(base, o62_twi, a.tw_inv), upload(base, o62_pc, a.pc),
+        upload(base, o62_pl, a.place),   upload(base, o62_cc, a.cc),      upload(base, o62_mq, a.miq),
+        hipMemcpy(base + o62_qc, &a.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
+        upload(base, o30_twf, b.tw_fwd), upload(base, o30_twi, b.tw_inv), upload(base, o30_pc, b.pc),
+        upload(base, o30_pl, b.place),   upload(base, o30_cc, b.cc),      upload(base, o30_mq, b.miq),
+        hipMemcpy(base + o30_qc, &b.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
     };
-    for (const Up& u : ups) {
-        e = hipMemcpy(base + u.off, u.src, u.bytes, hipMemcpyHostToDevice);
-        if (e != hipSuccess) return cleanup(hip_fail(e, "table upload"));
-    }
+    for (hipError_t u : ups)
+        if (u != hipSuccess) return cleanup(hip_fail(u, "table upload"));
     nc::DevPlan& d = p->dev;
     d.N = t.N;
     d.L = t.L;
-    d.k = t.k;
     d.log_n = t.log_n;
-    d.tw_fwd = reinterpret_cast<const nc::Tw*>(base + o_twf);
-    d.tw_inv = reinterpret_cast<const nc::Tw*>(base + o_twi);
-    d.pc = reinterpret_cast<const nc::PrimeConst*>(base + o_pc);
-    d.place = reinterpret_cast<const uint64_t*>(base + o_pl);
-    d.cc = reinterpret_cast<const nc::CrtConst*>(base + o_cc);
-    d.miq = reinterpret_cast<const uint32_t*>(base + o_mq);
-    d.qc = reinterpret_cast<const nc::QConst*>(base + o_qc);
+    d.w62.k = (uint32_t)a.primes.size();
+    d.w62.tw_fwd = reinterpret_cast<const nc::Tw*>(base + o62_twf);
+    d.w62.tw_inv = reinterpret_cast<const nc::Tw*>(base + o62_twi);
+    d.w62.pc = reinterpret_cast<const nc::PrimeConst*>(base + o62_pc);
+    d.w62.place = reinterpret_cast<const uint64_t*>(base + o62_pl);
+    d.w62.cc = reinterpret_cast<const nc::CrtConst*>(base + o62_cc);
+    d.w62.miq = reinterpret_cast<const uint32_t*>(base + o62_mq);
+    d.w62.qc = reinterpret_cast<const nc::QConst*>(base + o62_qc);
+    d.w30.k = (uint32_t)b.primes.size();
+    d.w30.tw_fwd = reinterpret_cast<const nc::Tw32*>(base + o30_twf);
+    d.w30.tw_inv = reinterpret_cast<const nc::Tw32*>(base + o30_twi);
+    d.w30.pc = reinterpret_cast<const nc::PrimeConst32*>(base + o30_pc);
+    d.w30.place = reinterpret_cast<const uint32_t*>(base + o30_pl);
+    d.w30.cc = reinterpret_cast<const nc::CrtConst32*>(base + o30_cc);
+    d.w30.miq = reinterpret_cast<const uint32_t*>(base + o30_mq);
+    d.w30.qc = reinterpret_cast<const nc::QConst*>(base + o30_qc);
+    p->res_bytes = std::max((size_t)d.w62.k * 8, (size_t)d.w30.k * 4) * t.N;
 
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hip_fail(e, "stream"));
@@ -199,50 +230,58 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     if (e != hipSuccess) return cleanup(hip_fail(e, "pinned staging"));
     int rc = grow_workspace(p, 1);
     if (rc) return cleanup(rc);
-    p->log_blocks = heuristic_log_blocks(t.log_n, t.k, 1);
-    p->crt_split = 1;
     *out = p;
     return NC_OK;
 }
 
 int nc_plan_get_info(const nc_plan* p, nc_plan_info* info) {
     if (!p || !info) return fail(NC_ERR_INVALID, "null argument");
+    const Choice c = choose(p, 1);
     info->N = p->host.N;
     info->L = p->host.L;
-    info->k = p->host.k;
     info->q_bits = p->host.q_bits;
-    const Choice c = choose(p, 1);
+    info->k62 = k_of(p, 62);
+    info->k30 = k_of(p, 30);
+    info->width = c.width;
     info->log_blocks = c.log_blocks;
-    info->crt_split = c.crt_split;
+    info->reserved = 0;
     info->table_bytes = p->table_bytes;
-    info->bytes_per_product = 2ull * p->host.k * p->host.N * sizeof(uint64_t);
+    info->bytes_per_product = 2ull * p->res_bytes;
     return NC_OK;
 }
 
-int nc_plan_primes(const nc_plan* p, uint64_t* out) {
+int nc_plan_primes(const nc_plan* p, uint32_t width, uint64_t* out) {
     if (!p || !out) return fail(NC_ERR_INVALID, "null argument");
-    std::memcpy(out, p->host.primes.data(), p->host.k * sizeof(uint64_t));
+    if (int rc = check_width(width)) return rc;
+    const std::vector<uint64_t>& v = width == 62 ? p->host.s62.primes : p->host.s30.primes;
+    std::memcpy(out, v.data(), v.size() * sizeof(uint64_t));
     return NC_OK;
 }
 
-int nc_plan_set_tuning(nc_plan* p, uint32_t log_blocks, uint32_t crt_split) {
+int nc_plan_set_tuning(nc_plan* p, uint32_t width, uint32_t log_blocks) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
-    if (log_blocks != 0xffffffffu) {
-        if (p->host.log_n >= 4 &&
-            (log_blocks < nc::min_log_blocks(p->host.log_n) || log_blocks > nc::max_log_blocks(p->host.log_n)))
-            return fail(NC_ERR_INVALID, "log_blocks out of range for this N");
-        p->log_blocks = p->host.log_n >= 4 ? log_blocks : 0;
-        p->forced = true;
+    if (width == 0) {  // back to autotuned / heuristic choices
+        p->forced = false;
+        return NC_OK;
     }
-    if (crt_split != 0xffffffffu) {
-        if (crt_split != 1 && crt_split != 4) return fail(NC_ERR_INVALID, "crt_split must be 1 or 4");
-        p->crt_split = crt_split;
-        p->forced = true;
+    if (int rc = check_width(width)) return rc;
+    Choice c;
+    c.width = width;
+    if (p->host.log_n >= 4) {
+        if (log_blocks == 0xffffffffu) {
+            c.log_blocks = heuristic(p, 1, width).log_blocks;
+        } else {
+            if (log_blocks < nc::min_log_blocks(p->host.log_n) || log_blocks > nc::max_log_blocks(p->host.log_n))
+                return fail(NC_ERR_INVALID, "log_blocks out of range for this N");
+            c.log_blocks = log_blocks;
+        }
     }
+    p->forced_choice = c;
+    p->forced = true;
     return NC_OK;
 }
 
-int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_log_blocks) {
+int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_width, uint32_t* chosen_log_blocks) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
     int rc = check_batch(p, batch);
     if (rc) return rc;
@@ -252,22 +291,25 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     const size_t n = (size_t)batch * p->host.N * p->host.L;
     uint32_t* buf = nullptr;
     NC_HIP(hipMalloc(&buf, 3 * n * sizeof(uint32_t)), "autotune buffers");
-    // zero inputs are valid (0 < q) and time exactly like any other input:
-    // every kernel's work is data independent
+    // zero inputs are valid (0 < q); the kernels' work does not depend on data
     hipError_t e = hipMemsetAsync(buf, 0, 2 * n * sizeof(uint32_t), s);
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
-    Choice best{heuristic_log_blocks(p->host.log_n, p->host.k, batch), 1u};
+    Choice best = heuristic(p, batch);
     float best_ms = -1.f;
-    if (p->host.log_n >= 4) {
-        const uint32_t lo = nc::min_log_blocks(p->host.log_n), hi = std::max(lo, nc::max_log_blocks(p->host.log_n));
+    const uint32_t log_n = p->host.log_n;
+    const uint32_t lo = log_n >= 4 ? nc::min_log_blocks(log_n) : 0u;
+    const uint32_t hi = log_n >= 4 ? std::max(lo, nc::max_log_blocks(log_n)) : 0u;
+    for (uint32_t width : {30u, 62u}) {
         for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb) {
-            const Choice c{lb, 1u};
+            Choice c;
+            c.width = width;
+            c.log_blocks = lb;
             float ms = 0.f;
             for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {  // rep 0 warms up
                 e = hipEventRecord(t0, s);
-                if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, &c) != NC_OK)
+                if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, c) != NC_OK)
                     e = hipErrorLaunchFailure;
                 if (e == hipSuccess) e = hipEventRecord(t1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(t1);
@@ -286,11 +328,11 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(e, "autotune");
-    nc_plan::Tuning& t = p->tune[bucket_of(batch)];
-    t.log_blocks = (uint8_t)best.log_blocks;
-    t.crt_split = (uint8_t)best.crt_split;
-    t.tuned = 1;
+    nc_plan::Tuned& t = p->tune[bucket_of(batch)];
+    t.choice = best;
+    t.tuned = true;
     p->forced = false;
+    if (chosen_width) *chosen_width = best.width;
     if (chosen_log_blocks) *chosen_log_blocks = best.log_blocks;
     return NC_OK;
 }
@@ -309,7 +351,7 @@ int nc_multiply(nc_plan* p, const uint32_t* a, const uint32_t* b, uint32_t* c, s
     std::memcpy(ha, a, bytes);
     std::memcpy(hb, b, bytes);
     NC_HIP(hipMemcpyAsync(da, ha, 2 * bytes, hipMemcpyHostToDevice, p->stream), "a/b copy");
-    int rc = run_pipeline(p, da, db, dc, 1, p->stream);
+    int rc = run_pipeline(p, da, db, dc, 1, p->stream, choose(p, 1));
     if (rc) return rc;
     NC_HIP(hipMemcpyAsync(hc, dc, bytes, hipMemcpyDeviceToHost, p->stream), "c copy");
     NC_HIP(hipStreamSynchronize(p->stream), "sync");
@@ -324,37 +366,40 @@ int nc_multiply_device(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uin
     if (rc) return rc;
     rc = grow_workspace(p, batch);
     if (rc) return rc;
-    return run_pipeline(p, d_a, d_b, d_c, batch, static_cast<hipStream_t>(stream));
+    return run_pipeline(p, d_a, d_b, d_c, batch, static_cast<hipStream_t>(stream), choose(p, batch));
 }
 
-int nc_stage_residues(nc_plan* p, const uint32_t* d_poly, uint64_t* d_res, uint32_t batch, void* stream) {
+int nc_stage_residues(nc_plan* p, uint32_t width, const uint32_t* d_poly, void* d_res, uint32_t batch, void* stream) {
     if (!p || !d_poly || !d_res) return fail(NC_ERR_INVALID, "null argument");
-    int rc = check_batch(p, batch);
-    if (rc) return rc;
-    NC_HIP(nc::launch_decompose(p->dev, d_poly, d_res, batch, nullptr, nullptr, 0, static_cast<hipStream_t>(stream)),
+    if (int rc = check_width(width)) return rc;
+    if (int rc = check_batch(p, batch)) return rc;
+    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_poly, d_res, batch, nullptr, nullptr, 0,
+                                static_cast<hipStream_t>(stream)),
            "decompose launch");
     return NC_OK;
 }
 
-int nc_stage_product_residues(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint64_t* d_res, uint32_t batch,
-                              void* stream) {
+int nc_stage_product_residues(nc_plan* p, uint32_t width, const uint32_t* d_a, const uint32_t* d_b, void* d_res,
+                              uint32_t batch, void* stream) {
     if (!p || !d_a || !d_b || !d_res) return fail(NC_ERR_INVALID, "null argument");
+    if (int rc = check_width(width)) return rc;
     int rc = check_batch(p, batch);
     if (rc) return rc;
     rc = grow_workspace(p, batch);
     if (rc) return rc;
+    Choice c = choose(p, batch);
+    if (c.width != width) c = heuristic(p, batch, width);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    NC_HIP(nc::launch_decompose(p->dev, d_a, d_res, batch, d_b, p->d_res_b, batch, s), "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, d_res, p->d_res_b, batch, choose(p, batch).log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s), "decompose launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
     return NC_OK;
 }
 
-int nc_stage_crt(nc_plan* p, const uint64_t* d_res, uint32_t* d_c, uint32_t batch, void* stream) {
+int nc_stage_crt(nc_plan* p, uint32_t width, const void* d_res, uint32_t* d_c, uint32_t batch, void* stream) {
     if (!p || !d_res || !d_c) return fail(NC_ERR_INVALID, "null argument");
-    int rc = check_batch(p, batch);
-    if (rc) return rc;
-    NC_HIP(nc::launch_crt(p->dev, d_res, d_c, batch, choose(p, batch).crt_split, static_cast<hipStream_t>(stream)),
-           "crt launch");
+    if (int rc = check_width(width)) return rc;
+    if (int rc = check_batch(p, batch)) return rc;
+    NC_HIP(nc::launch_crt(p->dev, (int)width, d_res, d_c, batch, static_cast<hipStream_t>(stream)), "crt launch");
     return NC_OK;
 }
 
@@ -367,21 +412,22 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Choice c = choose(p, batch);
+    const int w = (int)c.width;
     hipEvent_t ev[6];
     for (int i = 0; i < 6; ++i) NC_HIP(hipEventCreate(&ev[i]), "event");
     double acc[5] = {0, 0, 0, 0, 0};
     int status = NC_OK;
     for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
         hipError_t e = hipEventRecord(ev[0], s);
-        if (e == hipSuccess) e = nc::launch_decompose(p->dev, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s);
+        if (e == hipSuccess) e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s);
         if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-        if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-        if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-        if (e == hipSuccess) e = nc::launch_ntt_outer_inv(p->dev, p->d_res_a, batch, c.log_blocks, s);
+        if (e == hipSuccess) e = nc::launch_ntt_outer_inv(p->dev, w, p->d_res_a, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[4], s);
-        if (e == hipSuccess) e = nc::launch_crt(p->dev, p->d_res_a, d_c, batch, c.crt_split, s);
+        if (e == hipSuccess) e = nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s);
         if (e == hipSuccess) e = hipEventRecord(ev[5], s);
         if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
         for (int i = 0; i < 5 && e == hipSuccess; ++i) {
@@ -402,7 +448,9 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
 int nc_measure_mul32_peak(double* out) {
     if (!out) return fail(NC_ERR_INVALID, "null argument");
     hipDeviceProp_t prop;
-    NC_HIP(hipGetDeviceProperties(&prop, 0), "device properties");
+    int dev = 0;
+    NC_HIP(hipGetDevice(&dev), "device");
+    NC_HIP(hipGetDeviceProperties(&prop, dev), "device properties");
     const uint32_t blocks = (uint32_t)prop.multiProcessorCount * 16;  // 16 waves of 64 lanes per CU
     const uint32_t iters = 4096;
     uint32_t* d_in = nullptr;
@@ -439,7 +487,10 @@ int nc_gen_uniform(uint64_t seed, uint32_t N, uint32_t L, const uint32_t* q, uin
     if (!q || !out || L == 0 || L > (uint32_t)nc::kMaxLimbs || N == 0) return fail(NC_ERR_INVALID, "bad argument");
     int qbits = 0;
     for (int l = (int)L - 1; l >= 0; --l)
-        if (q[l]) { qbits = l * 32 + 32 - __builtin_clz(q[l]); break; }
+        if (q[l]) {
+            qbits = l * 32 + 32 - __builtin_clz(q[l]);
+            break;
+        }
     if (qbits == 0) return fail(NC_ERR_INVALID, "q is zero");
     const int top = qbits - 32 * ((int)L - 1);
     const uint32_t mask = top >= 32 ? 0xffffffffu : (top <= 0 ? 0u : ((1u << top) - 1u));

@@ -1,0 +1,84 @@
+// Modular word arithmetic for the gfx950 kernels, as two policies that the
+// NTT code is written against:
+//   W62: primes p < 2^62, 64-bit residues, 64x64 products built by the
+//        compiler from v_mad_u64_u32 / v_mul_hi_u32 / v_mul_lo_u32
+//   W30: primes p < 2^30, 32-bit residues, one v_mul_hi_u32 / v_mul_lo_u32
+//        per partial product
+// Both keep Harvey's lazy ranges: forward (Cooley-Tukey) values in [0, 4p),
+// inverse (Gentleman-Sande) values in [0, 2p), which needs 4p < 2^bits.
+// Twiddle products use Shoup's precomputed quotient ws = floor(w 2^bits / p);
+// the pointwise product is a Montgomery product (R = 2^bits).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "nc_common.hpp"
+
+namespace nc {
+
+using u128 = unsigned __int128;
+
+struct W62 {
+    using T = uint64_t;
+    using Prime = PrimeConst;
+    using Twid = Tw;
+    static constexpr int kBits = 64;
+
+    // a*w mod p in [0, 2p) for any a and w < p.
+    __device__ __forceinline__ static T shoup(T a, T w, T ws, T p) { return a * w - __umul64hi(a, ws) * p; }
+    __device__ __forceinline__ static T sub_if_ge(T x, T m) { return x >= m ? x - m : x; }
+    // a b 2^-64 mod p in [0, 2p) for a, b < 2p
+    __device__ __forceinline__ static T mont(T a, T b, T p, T mneg) {
+        const u128 t = (u128)a * b;
+        const uint64_t lo = (uint64_t)t, hi = (uint64_t)(t >> 64);
+        return hi + __umul64hi(lo * mneg, p) + (lo != 0);
+    }
+};
+
+struct W30 {
+    using T = uint32_t;
+    using Prime = PrimeConst32;
+    using Twid = Tw32;
+    static constexpr int kBits = 32;
+
+    __device__ __forceinline__ static T shoup(T a, T w, T ws, T p) { return a * w - __umulhi(a, ws) * p; }
+    // x - m if x >= m else x: when x < m, x - m wraps above x (m < 2^31), so
+    // the minimum picks the right one in two instructions.
+    __device__ __forceinline__ static T sub_if_ge(T x, T m) { return min(x, x - m); }
+    __device__ __forceinline__ static T mont(T a, T b, T p, T mneg) {
+        const uint64_t t = (uint64_t)a * b;
+        const uint32_t lo = (uint32_t)t, hi = (uint32_t)(t >> 32);
+        return hi + __umulhi(lo * mneg, p) + (lo != 0);
+    }
+};
+
+// Harvey butterflies on a policy A.
+template <class A>
+struct Bfly {
+    using T = typename A::T;
+    // CT: x, y in [0, 4p) -> (x + wy, x - wy) in [0, 4p)
+    __device__ __forceinline__ static void ct(T& x, T& y, typename A::Twid t, T p, T p2) {
+        const T u = A::sub_if_ge(x, p2);
+        const T v = A::shoup(y, t.w, t.ws, p);
+        x = u + v;
+        y = u - v + p2;
+    }
+    // GS: x, y in [0, 2p) -> (x + y, w (x - y)) in [0, 2p)
+    __device__ __forceinline__ static void gs(T& x, T& y, typename A::Twid t, T p, T p2) {
+        const T u = x + y;
+        const T d = x - y + p2;
+        x = A::sub_if_ge(u, p2);
+        y = A::shoup(d, t.w, t.ws, p);
+    }
+    // GS stage 0 with N^-1 R merged: x' = (x + y) nr, y' = (x - y) psi^-N/2 nr
+    __device__ __forceinline__ static void gs_last(T& x, T& y, const typename A::Prime& c) {
+        const T u = x + y;
+        const T d = x - y + c.p2;
+        x = A::shoup(u, c.nr, c.nr_s, c.p);
+        y = A::shoup(d, c.wr, c.wr_s, c.p);
+    }
+};
+
+}  // namespace nc

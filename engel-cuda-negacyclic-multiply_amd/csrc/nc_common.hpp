@@ -47,4 +47,29 @@ struct Tw {
     uint64_t ws;
 };
 
+// ---- 30-bit prime set (p < 2^30, 32-bit words) ----
+constexpr int kMaxPrimes30 = 72;
+
+struct PrimeConst32 {
+    uint32_t p, p2;
+    uint32_t mont_neg;  // -p^-1 mod 2^32
+    uint32_t nr, nr_s;  // N^-1 2^32 mod p, Shoup companion floor(nr 2^32 / p)
+    uint32_t wr, wr_s;  // psi^-bitrev(1) N^-1 2^32 mod p
+    uint32_t c32;       // 2^32 mod p (decomposition fold constant)
+};
+
+struct Tw32 {
+    uint32_t w;
+    uint32_t ws;  // floor(w 2^32 / p)
+};
+
+// CRT constants for a 30-bit prime: y = (M/p)^-1 mod p (Shoup pair) and
+// g = floor(2^96 / p) = g_hi 2^64 + g_lo for the fixed-point fraction
+// floor(v 2^64 / p) = floor(v g / 2^32) (at most one below).
+struct CrtConst32 {
+    uint32_t y, y_s;
+    uint32_t g_hi, pad;
+    uint64_t g_lo;
+};
+
 }  // namespace nc

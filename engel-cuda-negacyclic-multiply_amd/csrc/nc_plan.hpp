@@ -12,9 +12,8 @@
 
 namespace nc {
 
-// Host copy of the tables (also what the tests can inspect through the ABI).
-struct HostTables {
-    uint32_t N = 0, L = 0, k = 0, log_n = 0, q_bits = 0;
+// Host copy of one prime set's tables.
+struct HostSet62 {
     std::vector<uint64_t> primes;
     std::vector<Tw> tw_fwd, tw_inv;  // [k][N]
     std::vector<PrimeConst> pc;      // [k]
@@ -24,13 +23,35 @@ struct HostTables {
     QConst qc{};
 };
 
-// Builds the prime set (solve.cu:1174-1204 rule) and every table; throws
-// std::invalid_argument / std::runtime_error with a message on failure.
+struct HostSet30 {
+    std::vector<uint64_t> primes;
+    std::vector<Tw32> tw_fwd, tw_inv;
+    std::vector<PrimeConst32> pc;
+    std::vector<uint32_t> place;
+    std::vector<CrtConst32> cc;
+    std::vector<uint32_t> miq;
+    QConst qc{};
+};
+
+struct HostTables {
+    uint32_t N = 0, L = 0, log_n = 0, q_bits = 0;
+    HostSet62 s62;  // 62-bit primes: the reference's prime rule (solve.cu:1179-1204)
+    HostSet30 s30;  // 30-bit primes: the same rule below 2^30
+};
+
+// Builds both prime sets and every table; throws std::invalid_argument /
+// std::runtime_error with a message on failure.
 HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q);
 
-// Runs the init-time known-answer checks on the host copy (the analogue of
+// Init-time known-answer checks of both sets (the analogue of
 // solve.cu:1151-1166, 1285-1314, 1370-1415 for this formulation).
 void self_test(const HostTables& t);
+
+// Kernel configuration of one call.
+struct Choice {
+    uint32_t width = 62;     // prime width: 62 or 30
+    uint32_t log_blocks = 0; // NTT outer split
+};
 
 }  // namespace nc
 
@@ -41,24 +62,23 @@ struct nc_plan {
     // resident device tables (one allocation)
     void* d_tables = nullptr;
     size_t table_bytes = 0;
-    // workspace: residue rows of a and b for `ws_cap` products
-    uint64_t* d_res_a = nullptr;
-    uint64_t* d_res_b = nullptr;
+    // workspace: residue rows of a and b for `ws_cap` products, sized for
+    // the wider of the two prime sets
+    void* d_res_a = nullptr;
+    void* d_res_b = nullptr;
     uint32_t ws_cap = 0;
+    size_t res_bytes = 0;  // bytes of one product's residues (max over widths)
     // staging for the host-pointer path (one product)
-    uint32_t* d_io = nullptr;     // a | b | c, N*L u32 each
-    uint32_t* h_pin = nullptr;    // pinned a | b | c
+    uint32_t* d_io = nullptr;   // a | b | c, N*L u32 each
+    uint32_t* h_pin = nullptr;  // pinned a | b | c
     hipStream_t stream = nullptr;
-    // Tuning per batch-size bucket (bucket = ceil(log2 batch), 0..31): NTT
-    // outer split and CRT threads per coefficient.  `forced` pins every
-    // bucket to log_blocks / crt_split (nc_plan_set_tuning).
-    struct Tuning {
-        uint8_t log_blocks = 0;
-        uint8_t crt_split = 1;
-        uint8_t tuned = 0;
+    // Autotuned choice per batch-size bucket (ceil(log2 batch), 0..31);
+    // `forced` pins every bucket (nc_plan_set_tuning).
+    struct Tuned {
+        nc::Choice choice;
+        bool tuned = false;
     };
-    Tuning tune[32];
+    Tuned tune[32];
     bool forced = false;
-    uint32_t log_blocks = 0;      // forced / reported NTT outer split
-    uint32_t crt_split = 1;
+    nc::Choice forced_choice;
 };

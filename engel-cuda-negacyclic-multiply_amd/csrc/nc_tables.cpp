@@ -1,4 +1,4 @@
-// Plan tables: the prime set and every per-prime / per-q constant the
+// Plan tables: the prime sets and every per-prime / per-q constant the
 // kernels read.  Runs once per plan on the host (the reference does the same
 // work in fherma_init, solve.cu:1174-1398, outside the timed call).
 #include <cstring>
@@ -13,7 +13,9 @@ namespace nc {
 
 using namespace host;
 
-static uint32_t bitrev(uint32_t x, uint32_t bits) {
+namespace {
+
+uint32_t bitrev(uint32_t x, uint32_t bits) {
     uint32_t r = 0;
     for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
     return r;
@@ -21,7 +23,7 @@ static uint32_t bitrev(uint32_t x, uint32_t bits) {
 
 // Smallest base g >= 2 whose power g^((p-1)/2N) is a primitive 2N-th root
 // (psi^N = -1); the same choice rule as solve.cu:495-502.
-static uint64_t find_psi(uint64_t p, uint32_t N) {
+uint64_t find_psi(uint64_t p, uint32_t N) {
     const uint64_t e = (p - 1) / (2ull * N);
     for (uint64_t g = 2; g < 100000; ++g) {
         const uint64_t psi = powmod(g, e, p);
@@ -29,6 +31,195 @@ static uint64_t find_psi(uint64_t p, uint32_t N) {
     }
     throw std::runtime_error("no primitive 2N-th root of unity for prime " + std::to_string(p));
 }
+
+// Prime rule of solve.cu:1179-1204, parameterised by the word: candidates
+// p = 1 mod 2N descending from 2^bits - 1, taken while prime, until
+// bits(prod p) >= log2 N + 2 bits(q) + 4.
+std::vector<uint64_t> pick_primes(uint32_t N, uint32_t log_n, uint32_t q_bits, int bits, size_t max_k, Big& M) {
+    const uint32_t bound_bits = log_n + 2 * q_bits + 4;
+    const uint64_t step = 2ull * N;
+    uint64_t cand = (1ull << bits) - 1;
+    cand -= (cand - 1) % step;
+    M = Big::from_u64(1, 80);
+    std::vector<uint64_t> primes;
+    while (cand > step && (uint32_t)M.bits() < bound_bits) {
+        if (is_prime(cand)) {
+            if (primes.size() >= max_k) throw std::runtime_error("the coefficient bound does not fit the residue budget");
+            M.mul_u64(cand);
+            primes.push_back(cand);
+        }
+        cand -= step;
+    }
+    if ((uint32_t)M.bits() < bound_bits) throw std::runtime_error("not enough NTT primes for this modulus");
+    return primes;
+}
+
+// Twiddles psi^bitrev(x) / psi^-bitrev(x), x in [1, N), and the per-prime
+// values shared by both widths (as 64-bit numbers; the caller narrows).
+struct RootTables {
+    std::vector<uint64_t> fwd, inv;  // [N]
+    uint64_t nr = 0, wr = 0;         // N^-1 R mod p, psi^-bitrev(1) N^-1 R mod p
+};
+
+RootTables roots(uint64_t p, uint32_t N, uint32_t log_n, int word_bits) {
+    const uint64_t psi = find_psi(p, N);
+    const uint64_t psi_inv = invmod(psi, p);
+    std::vector<uint64_t> pw(N), ipw(N);
+    pw[0] = ipw[0] = 1;
+    for (uint32_t i = 1; i < N; ++i) {
+        pw[i] = mulmod(pw[i - 1], psi, p);
+        ipw[i] = mulmod(ipw[i - 1], psi_inv, p);
+    }
+    RootTables r;
+    r.fwd.assign(N, 0);
+    r.inv.assign(N, 0);
+    for (uint32_t x = 1; x < N; ++x) {
+        const uint32_t e = bitrev(x, log_n);
+        r.fwd[x] = pw[e];
+        r.inv[x] = ipw[e];
+    }
+    const uint64_t R = (uint64_t)((((u128)1) << word_bits) % p);  // the Montgomery radix mod p
+    r.nr = mulmod(invmod(N % p, p), R, p);
+    r.wr = mulmod(N > 1 ? r.inv[1] : 1, r.nr, p);
+    return r;
+}
+
+uint64_t shoup_w(uint64_t w, uint64_t p, int word_bits) { return (uint64_t)(((u128)w << word_bits) / p); }
+
+uint64_t neg_inv(uint64_t p, int word_bits) {
+    const uint64_t x = mont_neg_inv(p);  // -p^-1 mod 2^64
+    return word_bits == 64 ? x : (x & ((1ull << word_bits) - 1));
+}
+
+// CRT constants common to both widths: miq = (M/p_j) mod q, y_j, and the
+// QConst (q, (-M) mod q, Barrett constant).
+void crt_common(const std::vector<uint64_t>& primes, const Big& M, const Big& qb, uint32_t L, uint32_t q_bits,
+                std::vector<uint32_t>& miq, std::vector<uint64_t>& y, QConst& qc) {
+    const size_t k = primes.size();
+    miq.assign(k * kMaxLimbs, 0);
+    y.assign(k, 0);
+    for (size_t j = 0; j < k; ++j) {
+        const uint64_t p = primes[j];
+        Big Mj = Big::from_u64(1, 80);
+        uint64_t mj_mod_p = 1;
+        for (size_t o = 0; o < k; ++o) {
+            if (o == j) continue;
+            Mj.mul_u64(primes[o]);
+            mj_mod_p = mulmod(mj_mod_p, primes[o] % p, p);
+        }
+        y[j] = invmod(mj_mod_p, p);
+        Big r = mod(Mj, qb);
+        for (uint32_t l = 0; l < L; ++l) miq[j * kMaxLimbs + l] = r.v[l];
+    }
+    std::memset(&qc, 0, sizeof qc);
+    for (uint32_t l = 0; l < L; ++l) qc.q[l] = qb.v[l];
+    Big mq = mod(M, qb);
+    Big neg(L);
+    if (mq.bits() != 0) {
+        neg = qb;
+        sub_in(neg, mq);
+    }
+    for (uint32_t l = 0; l < L; ++l) qc.neg_m[l] = neg.v[l];
+    Big mu = pow2_div((int)q_bits - 1 + 72, qb, 4);
+    for (int i = 0; i < 4; ++i) qc.mu[i] = mu.v[i];
+    if (qc.mu[3] != 0) throw std::runtime_error("Barrett constant wider than 96 bits");
+    qc.q_bits = q_bits;
+    qc.L = L;
+}
+
+void build62(HostTables& t, const Big& qb) {
+    HostSet62& s = t.s62;
+    Big M;
+    s.primes = pick_primes(t.N, t.log_n, t.q_bits, 62, (size_t)kMaxPrimes, M);
+    const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
+    s.tw_fwd.resize((size_t)k * N);
+    s.tw_inv.resize((size_t)k * N);
+    s.pc.resize(k);
+    s.place.assign((size_t)k * kMaxLimbs, 0);
+    s.cc.resize(k);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint64_t p = s.primes[j];
+        RootTables r = roots(p, N, t.log_n, 64);
+        for (uint32_t x = 0; x < N; ++x) {
+            s.tw_fwd[(size_t)j * N + x] = Tw{r.fwd[x], x ? shoup_w(r.fwd[x], p, 64) : 0};
+            s.tw_inv[(size_t)j * N + x] = Tw{r.inv[x], x ? shoup_w(r.inv[x], p, 64) : 0};
+        }
+        PrimeConst& c = s.pc[j];
+        c.p = p;
+        c.p2 = 2 * p;
+        c.mont_neg = neg_inv(p, 64);
+        c.nr = r.nr;
+        c.nr_s = shoup_w(r.nr, p, 64);
+        c.wr = r.wr;
+        c.wr_s = shoup_w(r.wr, p, 64);
+        c.pad = 0;
+        uint64_t pl = (uint64_t)((((u128)1) << 64) % p);  // 2^64 mod p: cancelled by REDC
+        const uint64_t two32 = (1ull << 32) % p;
+        for (uint32_t l = 0; l < t.L; ++l) {
+            s.place[(size_t)j * kMaxLimbs + l] = pl;
+            pl = mulmod(pl, two32, p);
+        }
+    }
+    std::vector<uint64_t> y;
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint64_t p = s.primes[j];
+        CrtConst& cc = s.cc[j];
+        cc.y = y[j];
+        cc.y_s = shoup_w(y[j], p, 64);
+        const u128 mag = (~(u128)0) / p;  // floor(2^128 / p) (p odd, so 2^128 is no multiple)
+        cc.mag_hi = (uint64_t)(mag >> 64);
+        cc.mag_lo = (uint64_t)mag;
+    }
+}
+
+void build30(HostTables& t, const Big& qb) {
+    HostSet30& s = t.s30;
+    Big M;
+    s.primes = pick_primes(t.N, t.log_n, t.q_bits, 30, (size_t)kMaxPrimes30, M);
+    const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
+    s.tw_fwd.resize((size_t)k * N);
+    s.tw_inv.resize((size_t)k * N);
+    s.pc.resize(k);
+    s.place.assign((size_t)k * kMaxLimbs, 0);
+    s.cc.resize(k);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint64_t p = s.primes[j];
+        RootTables r = roots(p, N, t.log_n, 32);
+        for (uint32_t x = 0; x < N; ++x) {
+            s.tw_fwd[(size_t)j * N + x] = Tw32{(uint32_t)r.fwd[x], x ? (uint32_t)shoup_w(r.fwd[x], p, 32) : 0u};
+            s.tw_inv[(size_t)j * N + x] = Tw32{(uint32_t)r.inv[x], x ? (uint32_t)shoup_w(r.inv[x], p, 32) : 0u};
+        }
+        PrimeConst32& c = s.pc[j];
+        c.p = (uint32_t)p;
+        c.p2 = (uint32_t)(2 * p);
+        c.mont_neg = (uint32_t)neg_inv(p, 32);
+        c.nr = (uint32_t)r.nr;
+        c.nr_s = (uint32_t)shoup_w(r.nr, p, 32);
+        c.wr = (uint32_t)r.wr;
+        c.wr_s = (uint32_t)shoup_w(r.wr, p, 32);
+        c.c32 = (uint32_t)((1ull << 32) % p);
+        uint64_t pl = (1ull << 32) % p;  // 2^32 mod p: cancelled by the final REDC
+        for (uint32_t l = 0; l < t.L; ++l) {
+            s.place[(size_t)j * kMaxLimbs + l] = (uint32_t)pl;
+            pl = mulmod(pl, c.c32, p);
+        }
+    }
+    std::vector<uint64_t> y;
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint64_t p = s.primes[j];
+        CrtConst32& cc = s.cc[j];
+        cc.y = (uint32_t)y[j];
+        cc.y_s = (uint32_t)shoup_w(y[j], p, 32);
+        const u128 g = (((u128)1) << 96) / p;
+        cc.g_hi = (uint32_t)(g >> 64);
+        cc.pad = 0;
+        cc.g_lo = (uint64_t)g;
+    }
+}
+
+}  // namespace
 
 HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     // validation (solve.cu:1167-1172; N cap raised to 65536)
@@ -45,141 +236,65 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     if (t.q_bits < 2) throw std::invalid_argument("q must be an odd modulus > 1");
     // fast-CRT width (solve.cu:1339-1341): bits(q) + 70 <= 34 * 32
     if (t.q_bits + 70 > 1088) throw std::invalid_argument("q wider than 1018 bits is not supported");
-
-    // prime rule (solve.cu:1179-1204): p = 1 mod 2N, descending from 2^62 - 1,
-    // until bits(prod p) >= log2 N + 2 bits(q) + 4
-    const uint32_t bound_bits = t.log_n + 2 * t.q_bits + 4;
-    const uint64_t step = 2ull * N;
-    uint64_t cand = (1ull << 62) - 1;
-    cand -= (cand - 1) % step;
-    const size_t mlimbs = 80;
-    Big M = Big::from_u64(1, mlimbs);
-    while (cand > step && (uint32_t)M.bits() < bound_bits) {
-        if (is_prime(cand)) {
-            if (t.primes.size() >= (size_t)kMaxPrimes) throw std::runtime_error("the coefficient bound does not fit the residue budget");
-            M.mul_u64(cand);
-            t.primes.push_back(cand);
-        }
-        cand -= step;
-    }
-    if ((uint32_t)M.bits() < bound_bits) throw std::runtime_error("not enough NTT primes for this modulus");
-    const uint32_t k = (uint32_t)t.primes.size();
-    t.k = k;
-
-    t.tw_fwd.resize((size_t)k * N);
-    t.tw_inv.resize((size_t)k * N);
-    t.pc.resize(k);
-    t.place.assign((size_t)k * kMaxLimbs, 0);
-    t.cc.resize(k);
-    t.miq.assign((size_t)k * kMaxLimbs, 0);
-
-    std::vector<uint64_t> pw(N), ipw(N);
-    for (uint32_t j = 0; j < k; ++j) {
-        const uint64_t p = t.primes[j];
-        const uint64_t psi = find_psi(p, N);
-        const uint64_t psi_inv = invmod(psi, p);
-        pw[0] = ipw[0] = 1;
-        for (uint32_t i = 1; i < N; ++i) {
-            pw[i] = mulmod(pw[i - 1], psi, p);
-            ipw[i] = mulmod(ipw[i - 1], psi_inv, p);
-        }
-        Tw* tf = &t.tw_fwd[(size_t)j * N];
-        Tw* ti = &t.tw_inv[(size_t)j * N];
-        tf[0] = ti[0] = Tw{0, 0};
-        for (uint32_t x = 1; x < N; ++x) {
-            const uint32_t r = bitrev(x, t.log_n);
-            tf[x] = Tw{pw[r], shoup(pw[r], p)};
-            ti[x] = Tw{ipw[r], shoup(ipw[r], p)};
-        }
-        const uint64_t r64 = (uint64_t)(((u128)1 << 64) % p);  // 2^64 mod p
-        const uint64_t ninv = invmod(N % p, p);
-        PrimeConst& c = t.pc[j];
-        c.p = p;
-        c.p2 = 2 * p;
-        c.mont_neg = mont_neg_inv(p);
-        c.nr = mulmod(ninv, r64, p);
-        c.nr_s = shoup(c.nr, p);
-        const uint64_t last = N > 1 ? ti[1].w : 1;  // psi^-bitrev(1): the stage-0 inverse twiddle
-        c.wr = mulmod(last, c.nr, p);
-        c.wr_s = shoup(c.wr, p);
-        c.pad = 0;
-        // decomposition places 2^(32 l) 2^64 mod p (the 2^64 cancels in REDC)
-        uint64_t pl = r64;
-        const uint64_t two32 = (1ull << 32) % p;
-        for (uint32_t l = 0; l < L; ++l) {
-            t.place[(size_t)j * kMaxLimbs + l] = pl;
-            pl = mulmod(pl, two32, p);
-        }
-    }
-
-    // CRT constants (fast CRT, solve.cu:1342-1369 for the same quantities)
-    for (uint32_t j = 0; j < k; ++j) {
-        const uint64_t p = t.primes[j];
-        Big Mj = Big::from_u64(1, mlimbs);
-        uint64_t mj_mod_p = 1;
-        for (uint32_t o = 0; o < k; ++o) {
-            if (o == j) continue;
-            Mj.mul_u64(t.primes[o]);
-            mj_mod_p = mulmod(mj_mod_p, t.primes[o] % p, p);
-        }
-        CrtConst& cc = t.cc[j];
-        cc.y = invmod(mj_mod_p, p);
-        cc.y_s = shoup(cc.y, p);
-        const u128 mag = (~(u128)0) / p;  // floor((2^128 - 1)/p) = floor(2^128/p), p odd
-        cc.mag_hi = (uint64_t)(mag >> 64);
-        cc.mag_lo = (uint64_t)mag;
-        Big r = mod(Mj, qb);
-        for (uint32_t l = 0; l < L; ++l) t.miq[(size_t)j * kMaxLimbs + l] = r.v[l];
-    }
-    QConst& qc = t.qc;
-    std::memset(&qc, 0, sizeof qc);
-    for (uint32_t l = 0; l < L; ++l) qc.q[l] = q[l];
-    Big mq = mod(M, qb);
-    Big neg(L);
-    bool zero = mq.bits() == 0;
-    if (!zero) {
-        neg = qb;
-        sub_in(neg, mq);
-    }
-    for (uint32_t l = 0; l < L; ++l) qc.neg_m[l] = neg.v[l];
-    Big mu = pow2_div((int)t.q_bits - 1 + 72, qb, 4);
-    for (int i = 0; i < 4; ++i) qc.mu[i] = mu.v[i];
-    if (qc.mu[3] != 0) throw std::runtime_error("Barrett constant wider than 96 bits");
-    qc.q_bits = t.q_bits;
-    qc.L = L;
+    build62(t, qb);
+    build30(t, qb);
     return t;
 }
 
-// Known-answer checks of this formulation, run at plan creation.
+// Known-answer checks of both prime sets, run at plan creation (the
+// analogue of solve.cu:1151-1166, 1285-1314, 1370-1415 for this formulation).
 void self_test(const HostTables& t) {
     std::mt19937_64 rng(0x5eed);
-    for (uint32_t j = 0; j < t.k; ++j) {
-        const PrimeConst& c = t.pc[j];
-        const uint64_t p = c.p;
-        if (!is_prime(p) || (p - 1) % (2ull * t.N) != 0 || p >= (1ull << 62))
+    auto check_prime = [&](uint64_t p, uint64_t tw1, uint64_t tw1_inv, int bits) {
+        if (!is_prime(p) || (p - 1) % (2ull * t.N) != 0 || p >= (1ull << bits))
             throw std::runtime_error("self test: prime rule violated");
-        if ((uint64_t)(p * (0 - c.mont_neg)) != 1) throw std::runtime_error("self test: Montgomery constant");
-        // psi is a primitive 2N-th root: tw_fwd[1] = psi^(N/2) squares to -1
-        if (t.N > 1) {
-            const uint64_t h = t.tw_fwd[(size_t)j * t.N + 1].w;
-            if (mulmod(h, h, p) != p - 1) throw std::runtime_error("self test: psi order");
-            if (mulmod(h, t.tw_inv[(size_t)j * t.N + 1].w, p) != 1) throw std::runtime_error("self test: psi inverse");
+        if (t.N > 1) {  // tw[1] = psi^(N/2) squares to -1
+            if (mulmod(tw1, tw1, p) != p - 1) throw std::runtime_error("self test: psi order");
+            if (mulmod(tw1, tw1_inv, p) != 1) throw std::runtime_error("self test: psi inverse");
         }
-        // Shoup lazy product lands in [0, 2p) and agrees with the exact product
-        for (int trial = 0; trial < 16; ++trial) {
-            const uint64_t a = rng(), w = rng() % p;
-            const uint64_t ws = shoup(w, p);
+    };
+    uint32_t limbs[kMaxLimbs];
+    for (uint32_t j = 0; j < t.s62.primes.size(); ++j) {
+        const PrimeConst& c = t.s62.pc[j];
+        const uint64_t p = c.p;
+        check_prime(p, t.s62.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s62.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 62);
+        if ((uint64_t)(p * (0 - c.mont_neg)) != 1) throw std::runtime_error("self test: Montgomery constant");
+        for (int trial = 0; trial < 16; ++trial) {  // Shoup lazy product in [0, 2p), exact mod p
+            const uint64_t a = rng(), w = rng() % p, ws = shoup_w(w, p, 64);
             const uint64_t lazy = a * w - (uint64_t)(((u128)a * ws) >> 64) * p;
             if (lazy >= 2 * p || lazy % p != mulmod(a % p, w, p)) throw std::runtime_error("self test: Shoup product");
         }
-        // place table: REDC(sum limb 2^(32l) 2^64) equals Horner over the limbs
-        uint32_t limbs[kMaxLimbs];
         for (uint32_t l = 0; l < t.L; ++l) limbs[l] = (uint32_t)rng();
-        u128 s = 0;
-        for (uint32_t l = 0; l < t.L; ++l) s += (u128)limbs[l] * t.place[(size_t)j * kMaxLimbs + l];
-        const uint64_t lo = (uint64_t)s, hi = (uint64_t)(s >> 64);
-        const uint64_t m = lo * c.mont_neg;
-        uint64_t redc = hi + (uint64_t)(((u128)m * p) >> 64) + (lo != 0);
+        u128 s = 0;  // place table: REDC(sum limb 2^(32l) 2^64) equals Horner
+        for (uint32_t l = 0; l < t.L; ++l) s += (u128)limbs[l] * t.s62.place[(size_t)j * kMaxLimbs + l];
+        const uint64_t lo = (uint64_t)s;
+        uint64_t redc = (uint64_t)(s >> 64) + (uint64_t)(((u128)(lo * c.mont_neg) * p) >> 64) + (lo != 0);
+        if (redc >= p) redc -= p;
+        uint64_t horner = 0;
+        for (int l = (int)t.L - 1; l >= 0; --l) horner = (uint64_t)((((u128)horner) << 32 | limbs[l]) % p);
+        if (redc != horner) throw std::runtime_error("self test: digit sum does not match the modulus");
+    }
+    for (uint32_t j = 0; j < t.s30.primes.size(); ++j) {
+        const PrimeConst32& c = t.s30.pc[j];
+        const uint32_t p = c.p;
+        check_prime(p, t.s30.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s30.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 30);
+        if ((uint32_t)(p * (0u - c.mont_neg)) != 1u) throw std::runtime_error("self test: Montgomery constant");
+        for (int trial = 0; trial < 16; ++trial) {
+            const uint32_t a = (uint32_t)rng(), w = (uint32_t)(rng() % p), ws = (uint32_t)shoup_w(w, p, 32);
+            const uint32_t lazy = a * w - (uint32_t)(((uint64_t)a * ws) >> 32) * p;
+            if (lazy >= 2 * p || lazy % p != mulmod(a % p, w, p)) throw std::runtime_error("self test: Shoup product");
+        }
+        // decomposition fold + REDC equals Horner
+        for (uint32_t l = 0; l < t.L; ++l) limbs[l] = (uint32_t)rng();
+        uint64_t s = 0;
+        for (uint32_t l = 0; l < t.L; l += 2) {
+            s += (uint64_t)limbs[l] * t.s30.place[(size_t)j * kMaxLimbs + l];
+            if (l + 1 < t.L) s += (uint64_t)limbs[l + 1] * t.s30.place[(size_t)j * kMaxLimbs + l + 1];
+            s = (s >> 32) * c.c32 + (uint32_t)s;
+        }
+        s = (s >> 32) * c.c32 + (uint32_t)s;
+        const uint32_t lo = (uint32_t)s;
+        uint32_t redc = (uint32_t)(s >> 32) + (uint32_t)(((uint64_t)(lo * c.mont_neg) * p) >> 32) + (lo != 0);
         if (redc >= p) redc -= p;
         uint64_t horner = 0;
         for (int l = (int)t.L - 1; l >= 0; --l) horner = (uint64_t)((((u128)horner) << 32 | limbs[l]) % p);

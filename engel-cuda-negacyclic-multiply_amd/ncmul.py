@@ -36,9 +36,9 @@ class NcError(RuntimeError):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [
-        ("N", ctypes.c_uint32), ("L", ctypes.c_uint32), ("k", ctypes.c_uint32), ("q_bits", ctypes.c_uint32),
-        ("log_blocks", ctypes.c_uint32), ("crt_split", ctypes.c_uint32),
-        ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
+        ("N", ctypes.c_uint32), ("L", ctypes.c_uint32), ("q_bits", ctypes.c_uint32), ("k62", ctypes.c_uint32),
+        ("k30", ctypes.c_uint32), ("width", ctypes.c_uint32), ("log_blocks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
     ]
 
 
@@ -60,15 +60,16 @@ def lib():
         L.nc_plan_destroy.argtypes = [_vp]
         L.nc_plan_destroy.restype = None
         L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
-        L.nc_plan_primes.argtypes = [_vp, _u64p]
+        L.nc_plan_primes.argtypes = [_vp, ctypes.c_uint32, _u64p]
         L.nc_plan_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
         L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
         L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
-        L.nc_stage_residues.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
-        L.nc_stage_product_residues.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
-        L.nc_stage_crt.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_product_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_stage_crt.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
-        L.nc_plan_autotune.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.nc_plan_autotune.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32),
+                                       ctypes.POINTER(ctypes.c_uint32)]
         L.nc_profile_stages.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
                                         ctypes.POINTER(ctypes.c_float)]
         L.nc_measure_mul32_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -120,7 +121,7 @@ class Plan:
         _check(lib().nc_plan_create(self.N, self.L, _p32(self.q), ctypes.byref(h)), "nc_plan_create")
         self._h = h
         self.info = self.get_info()
-        self.k = self.info.k
+        self.k62, self.k30 = self.info.k62, self.info.k30
 
     def close(self):
         if getattr(self, "_h", None):
@@ -144,22 +145,27 @@ class Plan:
         _check(lib().nc_plan_get_info(self._h, ctypes.byref(info)), "nc_plan_get_info")
         return info
 
-    def primes(self) -> np.ndarray:
-        out = np.empty(self.k, dtype=np.uint64)
-        _check(lib().nc_plan_primes(self._h, out.ctypes.data_as(_u64p)), "nc_plan_primes")
+    def k(self, width: int) -> int:
+        return self.k62 if width == 62 else self.k30
+
+    def primes(self, width: int = 62) -> np.ndarray:
+        out = np.empty(self.k(width), dtype=np.uint64)
+        _check(lib().nc_plan_primes(self._h, width, out.ctypes.data_as(_u64p)), "nc_plan_primes")
         return out
 
-    def set_tuning(self, log_blocks=None, crt_split=None):
+    def set_tuning(self, width: int, log_blocks=None):
+        """Pin prime width (62 / 30) and NTT split; width 0 unpins."""
         lb = 0xFFFFFFFF if log_blocks is None else int(log_blocks)
-        cs = 0xFFFFFFFF if crt_split is None else int(crt_split)
-        _check(lib().nc_plan_set_tuning(self._h, lb, cs), "nc_plan_set_tuning")
+        _check(lib().nc_plan_set_tuning(self._h, int(width), lb), "nc_plan_set_tuning")
         self.info = self.get_info()
 
-    def autotune(self, batch: int, stream=None) -> int:
-        chosen = ctypes.c_uint32()
-        _check(lib().nc_plan_autotune(self._h, batch, _stream_handle(stream), ctypes.byref(chosen)),
+    def autotune(self, batch: int, stream=None):
+        """Returns the (prime width, NTT split) chosen for this batch size."""
+        w, lb = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().nc_plan_autotune(self._h, batch, _stream_handle(stream), ctypes.byref(w), ctypes.byref(lb)),
                "nc_plan_autotune")
-        return int(chosen.value)
+        self.info = self.get_info()
+        return int(w.value), int(lb.value)
 
     def profile_stages(self, d_a, d_b, d_c, batch: int, reps: int = 5, stream=None):
         """Mean per-launch milliseconds: decompose, ntt_outer_fwd, ntt_inner, ntt_outer_inv, crt."""
@@ -181,17 +187,17 @@ class Plan:
         _check(lib().nc_multiply_device(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()), _vp(d_c.data_ptr()),
                                         batch, _stream_handle(stream)), "nc_multiply_device")
 
-    def stage_residues(self, d_poly, d_res, batch: int, stream=None):
-        _check(lib().nc_stage_residues(self._h, _vp(d_poly.data_ptr()), _vp(d_res.data_ptr()), batch,
+    def stage_residues(self, width: int, d_poly, d_res, batch: int, stream=None):
+        _check(lib().nc_stage_residues(self._h, width, _vp(d_poly.data_ptr()), _vp(d_res.data_ptr()), batch,
                                        _stream_handle(stream)), "nc_stage_residues")
 
-    def stage_product_residues(self, d_a, d_b, d_res, batch: int, stream=None):
-        _check(lib().nc_stage_product_residues(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()),
+    def stage_product_residues(self, width: int, d_a, d_b, d_res, batch: int, stream=None):
+        _check(lib().nc_stage_product_residues(self._h, width, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()),
                                                _vp(d_res.data_ptr()), batch, _stream_handle(stream)),
                "nc_stage_product_residues")
 
-    def stage_crt(self, d_res, d_c, batch: int, stream=None):
-        _check(lib().nc_stage_crt(self._h, _vp(d_res.data_ptr()), _vp(d_c.data_ptr()), batch,
+    def stage_crt(self, width: int, d_res, d_c, batch: int, stream=None):
+        _check(lib().nc_stage_crt(self._h, width, _vp(d_res.data_ptr()), _vp(d_c.data_ptr()), batch,
                                   _stream_handle(stream)), "nc_stage_crt")
 
 

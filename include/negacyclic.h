@@ -12,11 +12,15 @@
  * reference stops at 32768, solve.cu:23).
  *
  * The arithmetic is exact: residue-number-system products over k NTT-friendly
- * primes below 2^62 chosen by the reference's prime rule
- * (solve.cu:1179-1204), negacyclic NTTs, a pointwise product and a centred
- * CRT back to L limbs mod q.  Every entry point returns NC_OK (0) or a
- * negative status; nc_last_error() gives the detail of the last failure on
- * the calling thread.
+ * primes, negacyclic NTTs, a pointwise product and a centred CRT back to L
+ * limbs mod q.  Two prime sets are built per plan, both by the reference's
+ * prime rule (solve.cu:1179-1204: p = 1 mod 2N, descending, until
+ * bits(prod p) >= log2 N + 2 bits(q) + 4): 62-bit primes below 2^62 (the
+ * reference's own set) and 30-bit primes below 2^30 (about twice as many,
+ * cheaper 32-bit arithmetic).  Which one a call uses is a tuning choice
+ * (nc_plan_autotune / nc_plan_set_tuning); the output never depends on it.
+ * Every entry point returns NC_OK (0) or a negative status; nc_last_error()
+ * gives the detail of the last failure on the calling thread.
  *
  * Threading: a plan is not re-entrant (like the reference's State,
  * solve.cu:28-65); use one plan per host thread, or serialise.
@@ -44,14 +48,16 @@ extern "C" {
 
 typedef struct nc_plan nc_plan;
 
-/* Plan information (nc_plan_info). */
+/* Plan information (nc_plan_get_info). */
 typedef struct nc_plan_info {
     uint32_t N;            /* ring degree                                   */
     uint32_t L;            /* limbs per coefficient                         */
-    uint32_t k;            /* number of RNS primes                          */
     uint32_t q_bits;       /* bits(q)                                       */
-    uint32_t log_blocks;   /* NTT outer split: log2(number of LDS blocks)   */
-    uint32_t crt_split;    /* threads per coefficient in the CRT kernel     */
+    uint32_t k62;          /* primes in the 62-bit set                      */
+    uint32_t k30;          /* primes in the 30-bit set                      */
+    uint32_t width;        /* prime width used for one product per call     */
+    uint32_t log_blocks;   /* its NTT outer split (log2 blocks per row)     */
+    uint32_t reserved;
     uint64_t table_bytes;  /* resident device bytes of the plan's tables    */
     uint64_t bytes_per_product; /* device workspace bytes per product       */
 } nc_plan_info;
@@ -71,19 +77,21 @@ const char* nc_last_error(void);
 int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out);
 void nc_plan_destroy(nc_plan* plan);
 int nc_plan_get_info(const nc_plan* plan, nc_plan_info* info);
-/* The k primes (host copy), out must hold info.k values. */
-int nc_plan_primes(const nc_plan* plan, uint64_t* out);
-/* Override the NTT outer split (log2 blocks per prime row, 0..4) and the
- * CRT threads-per-coefficient (1 or 4); 0xffffffff keeps the current value.
- * Any accepted setting yields bit-identical output. */
-int nc_plan_set_tuning(nc_plan* plan, uint32_t log_blocks, uint32_t crt_split);
+/* The primes of one set (width 62 or 30), host copy; out holds k values. */
+int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
+/* Pin every call to prime width `width` (62 or 30) and NTT outer split
+ * `log_blocks` (0xffffffff: a default for that width).  width 0 returns to
+ * the autotuned / default choices.  Any accepted setting yields bit-identical
+ * output. */
+int nc_plan_set_tuning(nc_plan* plan, uint32_t width, uint32_t log_blocks);
 
-/* Autotuner: times every NTT split for `batch` products on `stream` (zero
- * inputs; the kernels' work does not depend on the data) and keeps the
- * fastest for batches of that size bucket (ceil(log2 batch)).  Clears a
- * setting forced by nc_plan_set_tuning.  Output bits never depend on it.
- * (The reference has a single fixed schedule, solve.cu:1006-1038.) */
-int nc_plan_autotune(nc_plan* plan, uint32_t batch, void* stream, uint32_t* chosen_log_blocks);
+/* Autotuner: times every (prime width, NTT split) for `batch` products on
+ * `stream` (zero inputs; the kernels' work does not depend on the data) and
+ * keeps the fastest for batches of that size bucket (ceil(log2 batch)).
+ * Clears a setting forced by nc_plan_set_tuning.  Output bits never depend
+ * on it.  (The reference has a single fixed schedule, solve.cu:1006-1038.) */
+int nc_plan_autotune(nc_plan* plan, uint32_t batch, void* stream, uint32_t* chosen_width,
+                     uint32_t* chosen_log_blocks);
 
 /* ---- plain product, host buffers: replaces fherma_run ----------------
  * solve.cu:1489-1528: a, b in host memory (N*L u32 each), c written to host
@@ -100,19 +108,20 @@ int nc_multiply_device(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, 
                        void* stream);
 
 /* ---- stage entry points (device pointers, enqueue on stream) ---------
- * For stage-level parity checks against the CPU restatement.
- *  nc_stage_residues:          d_res[p][j][i] = poly_p[i] mod p_j (canonical)
+ * For stage-level parity checks.  `width` picks the prime set; residues are
+ * u64 for width 62 and u32 for width 30, layout [batch][k][N].
+ *  nc_stage_residues:          res[p][j][i] = poly_p[i] mod p_j (canonical)
  *                              (the value solve.cu:533-570 computes before
  *                              its Montgomery lift and twist)
- *  nc_stage_product_residues:  d_res[p][j][i] = c_int[i] mod p_j, c_int the
+ *  nc_stage_product_residues:  res[p][j][i] = c_int[i] mod p_j, c_int the
  *                              exact integer negacyclic product (canonical)
  *                              (solve.cu:1500-1514 up to untwist)
- *  nc_stage_crt:               d_c from canonical residues (solve.cu:752-836)
- * d_res holds batch x k x N u64. */
-int nc_stage_residues(nc_plan* plan, const uint32_t* d_poly, uint64_t* d_res, uint32_t batch, void* stream);
-int nc_stage_product_residues(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint64_t* d_res,
+ *  nc_stage_crt:               d_c from canonical residues (solve.cu:752-836) */
+int nc_stage_residues(nc_plan* plan, uint32_t width, const uint32_t* d_poly, void* d_res, uint32_t batch,
+                      void* stream);
+int nc_stage_product_residues(nc_plan* plan, uint32_t width, const uint32_t* d_a, const uint32_t* d_b, void* d_res,
                               uint32_t batch, void* stream);
-int nc_stage_crt(nc_plan* plan, const uint64_t* d_res, uint32_t* d_c, uint32_t batch, void* stream);
+int nc_stage_crt(nc_plan* plan, uint32_t width, const void* d_res, uint32_t* d_c, uint32_t batch, void* stream);
 
 /* ---- diagnostics -------------------------------------------------------
  * nc_profile_stages runs the batched pipeline `reps` times on `stream` and

@@ -141,6 +141,20 @@ def py_negacyclic(a_ints, b_ints, q: int) -> list:
     return [(lin[k] - lin[k + N]) % q for k in range(N)]
 
 
+def py_negacyclic_exact(a_ints, b_ints) -> list:
+    """The exact signed integer negacyclic product c_int (no reduction mod q),
+    by the same Kronecker substitution as py_negacyclic."""
+    N = len(a_ints)
+    bound = N * max(max(a_ints), 1) * max(max(b_ints), 1)
+    w = ((bound.bit_length() + 8) // 8) * 8
+    wb = w // 8
+    A = int.from_bytes(b"".join(int(x).to_bytes(wb, "little") for x in a_ints), "little")
+    B = int.from_bytes(b"".join(int(x).to_bytes(wb, "little") for x in b_ints), "little")
+    raw = (A * B).to_bytes(2 * N * wb, "little")
+    lin = [int.from_bytes(raw[wb * k: wb * (k + 1)], "little") for k in range(2 * N)]
+    return [lin[k] - lin[k + N] for k in range(N)]
+
+
 def py_negacyclic_loops(a_ints, b_ints, q: int) -> list:
     """Direct double loop over the definition (tiny N only)."""
     N = len(a_ints)

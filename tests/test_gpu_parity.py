@@ -1,10 +1,12 @@
 """GPU parity: the HIP library against the golden vectors and the oracle,
-bit for bit, through the C ABI (include/negacyclic.h).
+bit for bit, through the C ABI (include/negacyclic.h), for both prime sets
+(62-bit: the reference's prime rule; 30-bit: the same rule below 2^30).
 
 Sizes: every golden N (2 .. 65536) end to end; stage-level checks at
-N = 1024 and 16384; batch and tuning invariance; the reference's edge
-inputs by closed form; size-independent properties (commutativity,
-bilinearity, X^N = -1) at N = 65536.
+N = 1024 and 16384 (62-bit, against the CPU restatement of the reference)
+and N = 1024 (30-bit, against exact Python integers); batch and tuning
+invariance; the reference's edge inputs by closed form; size-independent
+properties (commutativity, bilinearity, X^N = -1) at N = 65536.
 """
 from __future__ import annotations
 
@@ -19,6 +21,8 @@ from conftest import load_golden, modulus
 
 pytestmark = pytest.mark.gpu
 
+WIDTHS = (62, 30)
+
 
 def sha(x):
     return hashlib.sha256(np.ascontiguousarray(x, dtype="<u4").tobytes()).hexdigest()
@@ -31,14 +35,16 @@ def say(*a):
 CASES = load_golden()["cases"]
 
 
+@pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['modulus']}-N{c['N']}-s{c['seed']}")
-def test_golden_host_path(gpu, case):
+def test_golden_host_path(gpu, case, width):
     """nc_multiply (the fherma_run replacement) against the Python-int vectors."""
     q = modulus(case["modulus"])
     N, s = case["N"], case["seed"]
-    say("golden", case["modulus"], N, s)
+    say("golden", case["modulus"], N, s, "width", width)
     a, b = ncmul.gen_uniform(2 * s, N, q), ncmul.gen_uniform(2 * s + 1, N, q)
     with ncmul.Plan(N, q) as plan:
+        plan.set_tuning(width)
         c = plan.multiply(a, b)
     assert sha(c) == case["sha256_c"]
     for idx, hx in case["samples"].items():
@@ -46,40 +52,73 @@ def test_golden_host_path(gpu, case):
 
 
 @pytest.mark.parametrize("N", [1024, 16384])
-def test_stage_residues(gpu, N, q868):
-    say("stage residues", N)
+def test_stage_residues_62(gpu, N, q868):
+    say("stage residues 62", N)
     a = ncmul.gen_uniform(21, N, q868)
     with ncmul.Plan(N, q868) as plan:
-        assert np.array_equal(plan.primes(), O.Plan(N, q868).primes())
+        assert np.array_equal(plan.primes(62), O.Plan(N, q868).primes())
         d_a = ncmul.to_device_u32(np.stack([a, a[::-1].copy()]))
-        d_res = ncmul.empty_device_u64((2, plan.k, N))
-        plan.stage_residues(d_a, d_res, 2)
+        d_res = ncmul.empty_device_u64((2, plan.k62, N))
+        plan.stage_residues(62, d_a, d_res, 2)
         got = ncmul.from_device_u64(d_res)
     orc = O.Plan(N, q868)
     assert np.array_equal(got[0], orc.residues(a))
     assert np.array_equal(got[1], orc.residues(a[::-1].copy()))
 
 
+def test_stage_residues_30(gpu, q868):
+    N = 1024
+    say("stage residues 30", N)
+    a = ncmul.gen_uniform(22, N, q868)
+    ai = O.poly_to_ints(a)
+    with ncmul.Plan(N, q868) as plan:
+        pr = [int(p) for p in plan.primes(30)]
+        assert all(p < (1 << 30) and p % (2 * N) == 1 for p in pr)
+        d_a = ncmul.to_device_u32(a[None])
+        d_res = ncmul.empty_device_u32((1, plan.k30, N))
+        plan.stage_residues(30, d_a, d_res, 1)
+        got = ncmul.from_device_u32(d_res)[0]
+    for j, p in enumerate(pr):
+        assert [int(x) for x in got[j]] == [x % p for x in ai], j
+
+
 @pytest.mark.parametrize("N", [1024, 16384])
-def test_stage_product_residues_and_crt(gpu, N, q868):
-    say("stage product residues", N)
+def test_stage_product_residues_and_crt_62(gpu, N, q868):
+    say("stage product residues 62", N)
     a, b = ncmul.gen_uniform(31, N, q868), ncmul.gen_uniform(32, N, q868)
     orc = O.Plan(N, q868)
     want = orc.product_residues(a, b)
     with ncmul.Plan(N, q868) as plan:
         d_a, d_b = ncmul.to_device_u32(a[None]), ncmul.to_device_u32(b[None])
-        d_res = ncmul.empty_device_u64((1, plan.k, N))
-        plan.stage_product_residues(d_a, d_b, d_res, 1)
-        got = ncmul.from_device_u64(d_res)[0]
-        assert np.array_equal(got, want)
-        # CRT alone, fed the oracle's residues
+        d_res = ncmul.empty_device_u64((1, plan.k62, N))
+        plan.stage_product_residues(62, d_a, d_b, d_res, 1)
+        assert np.array_equal(ncmul.from_device_u64(d_res)[0], want)
         d_c = ncmul.empty_device_u32((1, N, 28))
-        plan.stage_crt(ncmul.to_device_u64(want[None]), d_c, 1)
+        plan.stage_crt(62, ncmul.to_device_u64(want[None]), d_c, 1)
         assert np.array_equal(ncmul.from_device_u32(d_c)[0], orc.crt_fast(want))
 
 
+def test_stage_product_residues_and_crt_30(gpu, q868):
+    N = 1024
+    say("stage product residues 30", N)
+    qi = O.limbs_to_int(q868)
+    a, b = ncmul.gen_uniform(33, N, q868), ncmul.gen_uniform(34, N, q868)
+    cint = O.py_negacyclic_exact(O.poly_to_ints(a), O.poly_to_ints(b))
+    with ncmul.Plan(N, q868) as plan:
+        pr = [int(p) for p in plan.primes(30)]
+        want = np.array([[c % p for c in cint] for p in pr], dtype=np.uint32)
+        d_a, d_b = ncmul.to_device_u32(a[None]), ncmul.to_device_u32(b[None])
+        d_res = ncmul.empty_device_u32((1, plan.k30, N))
+        plan.stage_product_residues(30, d_a, d_b, d_res, 1)
+        assert np.array_equal(ncmul.from_device_u32(d_res)[0], want)
+        d_c = ncmul.empty_device_u32((1, N, 28))
+        plan.stage_crt(30, ncmul.to_device_u32(want[None]), d_c, 1)
+        assert O.poly_to_ints(ncmul.from_device_u32(d_c)[0]) == [c % qi for c in cint]
+
+
+@pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("N", [16, 1024, 16384, 65536])
-def test_tuning_invariance(gpu, N, q868):
+def test_tuning_invariance(gpu, N, width, q868):
     """Every accepted NTT split gives the same bits as the oracle."""
     a, b = ncmul.gen_uniform(41, N, q868), ncmul.gen_uniform(42, N, q868)
     want = O.Plan(N, q868).multiply(a, b)
@@ -87,30 +126,43 @@ def test_tuning_invariance(gpu, N, q868):
         logn = N.bit_length() - 1
         lo, hi = max(0, logn - 14), min(4, logn - 4)
         for lb in range(lo, hi + 1):
-            say("tuning", N, lb)
-            plan.set_tuning(log_blocks=lb)
+            say("tuning", N, width, lb)
+            plan.set_tuning(width, lb)
             assert np.array_equal(plan.multiply(a, b), want), lb
 
 
+@pytest.mark.parametrize("width", WIDTHS)
 @pytest.mark.parametrize("N", [1024, 16384])
-def test_device_batch(gpu, N, q868):
-    say("batch", N)
+def test_device_batch(gpu, N, width, q868):
+    say("batch", N, width)
     B = 5
     a = np.stack([ncmul.gen_uniform(100 + 2 * i, N, q868) for i in range(B)])
     b = np.stack([ncmul.gen_uniform(101 + 2 * i, N, q868) for i in range(B)])
     orc = O.Plan(N, q868)
     with ncmul.Plan(N, q868) as plan:
+        plan.set_tuning(width)
         d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
         d_c = ncmul.empty_device_u32((B, N, 28))
         plan.multiply_device(d_a, d_b, d_c, B)
         got = ncmul.from_device_u32(d_c)
         for i in range(B):
             assert np.array_equal(got[i], orc.multiply(a[i], b[i])), i
-        # second call with a smaller batch reuses the workspace
-        plan.multiply_device(d_b, d_a, d_c, 2)
+        plan.multiply_device(d_b, d_a, d_c, 2)  # smaller batch reuses the workspace
         got2 = ncmul.from_device_u32(d_c)
         for i in range(2):
             assert np.array_equal(got2[i], got[i])
+
+
+def test_autotune_keeps_bits(gpu, q868):
+    N = 4096
+    a, b = ncmul.gen_uniform(61, N, q868), ncmul.gen_uniform(62, N, q868)
+    want = O.Plan(N, q868).multiply(a, b)
+    with ncmul.Plan(N, q868) as plan:
+        for batch in (1, 8):
+            w, lb = plan.autotune(batch)
+            say("autotune", batch, w, lb)
+            assert w in WIDTHS
+        assert np.array_equal(plan.multiply(a, b), want)
 
 
 @pytest.mark.parametrize("N", [2, 1024, 16384, 65536])
@@ -118,35 +170,33 @@ def test_edges_closed_form(gpu, N, q868):
     say("edges", N)
     qi = O.limbs_to_int(q868)
     with ncmul.Plan(N, q868) as plan:
-        full = np.tile(O.int_to_limbs(qi - 1, 28), (N, 1))
-        c = plan.multiply(full, full)
-        ks = sorted({0, 1, N // 2, N - 1})
-        for k in ks:
-            assert O.limbs_to_int(c[k]) == (2 * k + 2 - N) % qi
-        zero = np.zeros((N, 28), dtype=np.uint32)
-        assert not plan.multiply(zero, full).any()
-        a = zero.copy()
-        b = zero.copy()
-        v = (qi * 5) // 11
-        a[N - 1] = O.int_to_limbs(v, 28)
-        b[1 % N] = O.int_to_limbs(1, 28)
-        c = plan.multiply(a, b)
-        if N > 1:
+        for width in WIDTHS:
+            plan.set_tuning(width)
+            full = np.tile(O.int_to_limbs(qi - 1, 28), (N, 1))
+            c = plan.multiply(full, full)
+            for k in sorted({0, 1, N // 2, N - 1}):
+                assert O.limbs_to_int(c[k]) == (2 * k + 2 - N) % qi
+            zero = np.zeros((N, 28), dtype=np.uint32)
+            assert not plan.multiply(zero, full).any()
+            a, b = zero.copy(), zero.copy()
+            v = (qi * 5) // 11
+            a[N - 1] = O.int_to_limbs(v, 28)
+            b[1] = O.int_to_limbs(1, 28)
+            c = plan.multiply(a, b)
             assert O.limbs_to_int(c[0]) == qi - v
             assert not c[1:].any()
-        x = ncmul.gen_uniform(7, N, q868)
-        one = zero.copy()
-        one[0] = O.int_to_limbs(1, 28)
-        assert np.array_equal(plan.multiply(x, one), x)
-        neg = zero.copy()
-        neg[0] = O.int_to_limbs(qi - 1, 28)
-        cx = plan.multiply(x, neg)
-        assert O.poly_to_ints(cx) == [(qi - y) % qi for y in O.poly_to_ints(x)]
+            x = ncmul.gen_uniform(7, N, q868)
+            one = zero.copy()
+            one[0] = O.int_to_limbs(1, 28)
+            assert np.array_equal(plan.multiply(x, one), x)
+            neg = zero.copy()
+            neg[0] = O.int_to_limbs(qi - 1, 28)
+            assert O.poly_to_ints(plan.multiply(x, neg)) == [(qi - y) % qi for y in O.poly_to_ints(x)]
 
 
 def test_properties_max_size(gpu, q868):
     """N = 65536 (beyond the reference's cap): commutativity, bilinearity and
-    the X^N = -1 shift, all exact."""
+    the X^N = -1 shift, all exact, on the default configuration."""
     N = 65536
     qi = O.limbs_to_int(q868)
     say("properties", N)
@@ -158,7 +208,6 @@ def test_properties_max_size(gpu, q868):
         lhs = O.poly_to_ints(plan.multiply(s, b))
         rhs = [(x + y) % qi for x, y in zip(O.poly_to_ints(ab), O.poly_to_ints(plan.multiply(a2, b)))]
         assert lhs == rhs
-        # b * X: coefficients shift up by one, the top one wraps negated
         x1 = np.zeros_like(a)
         x1[1] = O.int_to_limbs(1, 28)
         sh = O.poly_to_ints(plan.multiply(a, x1))
@@ -173,6 +222,6 @@ def test_errors(gpu, q868):
             plan.multiply(a[:-1], a[:-1])
         assert e.value.status == -2
         with pytest.raises(ncmul.NcError):
-            plan.set_tuning(log_blocks=9)
+            plan.set_tuning(62, 9)
         with pytest.raises(ncmul.NcError):
-            plan.set_tuning(crt_split=3)
+            plan.set_tuning(31)
