@@ -74,14 +74,15 @@ def bytes_per_stage(N: int, L: int, k: int, word_bytes: int, batch: int) -> dict
     }
 
 
-def pmc_traffic(kernel: str):
+def pmc_traffic(kernel: str, N: int, width: int, split: int, batch: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (profiles/*pmc*.md), or None when no such summary exists."""
-    for f in sorted((ROOT / "profiles").glob("*pmc*.md")):
-        m = re.search(rf"^traffic_bytes_per_launch\s+{kernel}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
-        if m:
-            return float(m.group(1)), f.name
-    return None, None
+    summary for this exact configuration
+    (profiles/pmc_traffic_N{N}_w{width}_s{split}_b{batch}.md), or None."""
+    f = ROOT / "profiles" / f"pmc_traffic_N{N}_w{width}_s{split}_b{batch}.md"
+    if not f.exists():
+        return None, None
+    m = re.search(rf"^traffic_bytes_per_launch\s+{kernel}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+    return (float(m.group(1)), f.name) if m else (None, None)
 
 
 def cpu_baseline(N: int, q: np.ndarray, seconds: float):
@@ -118,6 +119,11 @@ def main() -> int:
     ap.add_argument("--plain-calls", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="force 'width,split' for both plans instead of autotuning (e.g. 30,2)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (a known-size kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -143,10 +149,25 @@ def main() -> int:
     d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
     d_c = ncmul.empty_device_u32((B, N, L))
 
+    if args.calibrate_copy_mb:
+        n = args.calibrate_copy_mb << 18  # int32 elements
+        src = torch.ones(n, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     batched = ncmul.Plan(N, q)
-    w_batch, s1_batch = batched.autotune(B)
     plain = ncmul.Plan(N, q)
-    w_plain, s1_plain = plain.autotune(1)
+    if args.tuning:
+        w, s1 = (int(x) for x in args.tuning.split(","))
+        batched.set_tuning(w, s1)
+        plain.set_tuning(w, s1)
+        w_batch, s1_batch, w_plain, s1_plain = w, s1, w, s1
+    else:
+        w_batch, s1_batch = batched.autotune(B)
+        w_plain, s1_plain = plain.autotune(1)
 
     # ---- batched: the timed steps ----
     for _ in range(args.warmup):
@@ -198,7 +219,7 @@ def main() -> int:
     peak_mul = ncmul.measure_mul32_peak()
     dom = max(stage_ms, key=lambda s: stage_ms[s])
     dom_s = stage_ms[dom] * 1e-3
-    traffic, traffic_src = pmc_traffic(dom)
+    traffic, traffic_src = pmc_traffic(dom, N, w_batch, s1_batch, B)
     roofline = {
         "bound": "valu",
         "kernel": dom,
