@@ -48,10 +48,13 @@ uint32_t k_of(const nc_plan* p, uint32_t width) {
 // Untuned default: 30-bit primes, and the smallest outer split that still
 // gives about 1024 workgroup-sized blocks (k * batch * 2^s1): fewer, larger
 // blocks cost fewer HBM round trips but must still fill 256 compute units.
+bool mfma_default(const nc_plan* p) { return p->mfma_mode != 0; }
+
 Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
     const uint32_t log_n = p->host.log_n;
     Choice c;
     c.width = width;
+    c.mfma = mfma_default(p);
     if (log_n < 4) return c;
     const uint32_t lo = nc::min_log_blocks(log_n), hi = std::max(lo, nc::max_log_blocks(log_n));
     uint32_t s = lo;
@@ -61,10 +64,13 @@ Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
 }
 
 Choice choose(const nc_plan* p, uint32_t batch) {
-    if (p->forced) return p->forced_choice;
+    Choice c;
     const nc_plan::Tuned& t = p->tune[bucket_of(batch)];
-    if (t.tuned) return t.choice;
-    return heuristic(p, batch);
+    if (p->forced) c = p->forced_choice;
+    else if (t.tuned) c = t.choice;
+    else c = heuristic(p, batch);
+    if (p->mfma_mode >= 0) c.mfma = p->mfma_mode == 1;
+    return c;
 }
 
 int grow_workspace(nc_plan* p, uint32_t batch) {
@@ -94,7 +100,7 @@ int check_width(uint32_t width) {
 
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
-    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s),
+    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, c.mfma),
            "decompose launch");
     NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
     NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s), "crt launch");
@@ -185,6 +191,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     const size_t o30_pc = slot(b.pc.size() * sizeof(nc::PrimeConst32)), o30_pl = slot(b.place.size() * 4);
     const size_t o30_cc = slot(b.cc.size() * sizeof(nc::CrtConst32)), o30_mq = slot(b.miq.size() * 4);
     const size_t o30_qc = slot(sizeof(nc::QConst));
+    const size_t o30_mt = slot(b.mfma_tab.size()), o30_mg = slot(b.mfma_g.size() * sizeof(int64_t));
     p->table_bytes = off;
     e = hipMalloc(&p->d_tables, off);
     if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
@@ -196,6 +203,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         upload(base, o30_twf, b.tw_fwd), upload(base, o30_twi, b.tw_inv), upload(base, o30_pc, b.pc),
         upload(base, o30_pl, b.place),   upload(base, o30_cc, b.cc),      upload(base, o30_mq, b.miq),
         hipMemcpy(base + o30_qc, &b.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
+        upload(base, o30_mt, b.mfma_tab), upload(base, o30_mg, b.mfma_g),
     };
     for (hipError_t u : ups)
         if (u != hipSuccess) return cleanup(hip_fail(u, "table upload"));
@@ -219,6 +227,11 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     d.w30.cc = reinterpret_cast<const nc::CrtConst32*>(base + o30_cc);
     d.w30.miq = reinterpret_cast<const uint32_t*>(base + o30_mq);
     d.w30.qc = reinterpret_cast<const nc::QConst*>(base + o30_qc);
+    if (!b.mfma_tab.empty()) {
+        d.w30.mfma_tab = reinterpret_cast<const int8_t*>(base + o30_mt);
+        d.w30.mfma_g = reinterpret_cast<const int64_t*>(base + o30_mg);
+        d.w30.mfma_tiles = (uint32_t)(b.mfma_tab.size() / (32 * nc::kMfmaK));
+    }
     p->res_bytes = std::max((size_t)d.w62.k * 8, (size_t)d.w30.k * 4) * t.N;
 
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
@@ -244,7 +257,7 @@ int nc_plan_get_info(const nc_plan* p, nc_plan_info* info) {
     info->k30 = k_of(p, 30);
     info->width = c.width;
     info->log_blocks = c.log_blocks;
-    info->reserved = 0;
+    info->decompose_mfma = (c.width == 30 && c.mfma && p->dev.w30.mfma_tab && p->host.N >= 32) ? 1u : 0u;
     info->table_bytes = p->table_bytes;
     info->bytes_per_product = 2ull * p->res_bytes;
     return NC_OK;
@@ -267,6 +280,7 @@ int nc_plan_set_tuning(nc_plan* p, uint32_t width, uint32_t log_blocks) {
     if (int rc = check_width(width)) return rc;
     Choice c;
     c.width = width;
+    c.mfma = mfma_default(p);
     if (p->host.log_n >= 4) {
         if (log_blocks == 0xffffffffu) {
             c.log_blocks = heuristic(p, 1, width).log_blocks;
@@ -278,6 +292,25 @@ int nc_plan_set_tuning(nc_plan* p, uint32_t width, uint32_t log_blocks) {
     }
     p->forced_choice = c;
     p->forced = true;
+    return NC_OK;
+}
+
+int nc_plan_get_choice(const nc_plan* p, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
+                       uint32_t* decompose_mfma) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
+    const Choice c = choose(p, batch);
+    if (width) *width = c.width;
+    if (log_blocks) *log_blocks = c.log_blocks;
+    if (decompose_mfma)
+        *decompose_mfma = (c.width == 30 && c.mfma && p->dev.w30.mfma_tab && p->host.N >= 32) ? 1u : 0u;
+    return NC_OK;
+}
+
+int nc_plan_set_mfma(nc_plan* p, int mode) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (mode < -1 || mode > 1) return fail(NC_ERR_INVALID, "mode must be -1, 0 or 1");
+    p->mfma_mode = mode;
     return NC_OK;
 }
 
@@ -301,10 +334,25 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     const uint32_t log_n = p->host.log_n;
     const uint32_t lo = log_n >= 4 ? nc::min_log_blocks(log_n) : 0u;
     const uint32_t hi = log_n >= 4 ? std::max(lo, nc::max_log_blocks(log_n)) : 0u;
+    const bool mfma_avail = p->dev.w30.mfma_tab != nullptr && p->host.N >= 32;
+    struct Cand {
+        uint32_t width;
+        bool mfma;
+    };
+    std::vector<Cand> cands;
     for (uint32_t width : {30u, 62u}) {
+        if (width == 30 && mfma_avail && p->mfma_mode < 0) {
+            cands.push_back({30u, true});
+            cands.push_back({30u, false});
+        } else {
+            cands.push_back({width, mfma_default(p)});
+        }
+    }
+    for (const Cand& cd : cands) {
         for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb) {
             Choice c;
-            c.width = width;
+            c.width = cd.width;
+            c.mfma = cd.mfma;
             c.log_blocks = lb;
             float ms = 0.f;
             for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {  // rep 0 warms up
@@ -374,7 +422,7 @@ int nc_stage_residues(nc_plan* p, uint32_t width, const uint32_t* d_poly, void* 
     if (int rc = check_width(width)) return rc;
     if (int rc = check_batch(p, batch)) return rc;
     NC_HIP(nc::launch_decompose(p->dev, (int)width, d_poly, d_res, batch, nullptr, nullptr, 0,
-                                static_cast<hipStream_t>(stream)),
+                                static_cast<hipStream_t>(stream), choose(p, batch).mfma),
            "decompose launch");
     return NC_OK;
 }
@@ -390,7 +438,8 @@ int nc_stage_product_residues(nc_plan* p, uint32_t width, const uint32_t* d_a, c
     Choice c = choose(p, batch);
     if (c.width != width) c = heuristic(p, batch, width);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s), "decompose launch");
+    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s, c.mfma),
+           "decompose launch");
     NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
     return NC_OK;
 }
@@ -419,7 +468,8 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     int status = NC_OK;
     for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
         hipError_t e = hipEventRecord(ev[0], s);
-        if (e == hipSuccess) e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s);
+        if (e == hipSuccess)
+            e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, c.mfma);
         if (e == hipSuccess) e = hipEventRecord(ev[1], s);
         if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[2], s);

@@ -146,6 +146,94 @@ __global__ __launch_bounds__(256) void decompose30_kernel(const uint32_t* __rest
     }
 }
 
+// 30-bit primes on the matrix cores: the digit sum is an int8 contraction.
+// With a_m the 4L bytes of a coefficient and t_um byte u of
+// T_m = 2^(8m) 2^32 mod p_j (table rows 4j + u),
+//   V_j = sum_m a_m T_m = sum_u 2^(8u) X_u,   X_u = sum_m a_m t_um.
+// The MFMA computes D_u = sum_m (a_m - 128)(t_um - 128) (signed int8 inputs,
+// exact int32 sums), and X_u = D_u + 128 Sa + 128 sum_m t_um - 4L 128^2 with
+// Sa = sum_m a_m, so V_j = sum_u 2^(8u) D_u + 128 Sa 0x01010101 + G_j, G_j
+// precomputed.  0 <= V_j < 4L 255 2^30 < p 2^32, and one REDC (R = 2^32)
+// gives the residue (the table carries the 2^32).
+// Layout (v_mfma_i32_32x32x32_i8): the A operand is the table (rows =
+// prime x byte, 8 primes per 32-row tile), the B operand is 32 coefficients'
+// bytes, so lane l receives all four byte sums of primes 2h + l/32 (h = 0..3)
+// for coefficient l % 32 and recombines them without any lane exchange.
+typedef int nc_v4i __attribute__((ext_vector_type(4)));
+typedef int nc_v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kMfmaCoeffTiles = 4;  // 32-coefficient tiles per wave
+
+__global__ __launch_bounds__(256) void decompose30_mfma_kernel(
+    const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
+    uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int64_t* __restrict__ gtab,
+    const PrimeConst32* __restrict__ pc, uint32_t N, uint32_t L, uint32_t k, uint32_t tiles) {
+    __shared__ int8_t lds_tab[kMfmaTiles * 32 * kMfmaRowStride];
+    struct PrimeLds {
+        int64_t g;
+        uint32_t p, mont_neg;
+    };
+    __shared__ PrimeLds lds_prime[kMfmaTiles * 8];
+    constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
+    for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
+        const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
+        *reinterpret_cast<nc_v4i*>(lds_tab + row * kMfmaRowStride + chunk * 16) =
+            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
+    }
+    for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256)
+        lds_prime[j] = j < k ? PrimeLds{gtab[j], pc[j].p, pc[j].mont_neg} : PrimeLds{0, 1u, 0u};
+    __syncthreads();
+    const uint32_t wgs_per_poly = (N + kPerWg - 1) / kPerWg;
+    const uint32_t poly = blockIdx.x / wgs_per_poly;
+    const uint32_t wg_base = (blockIdx.x - poly * wgs_per_poly) * kPerWg;
+    const bool is_a = poly < na;
+    const uint32_t pi = is_a ? poly : poly - na;
+    const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
+    uint32_t* dst = (is_a ? res_a : res_b) + (size_t)pi * k * N;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 31u, half = lane >> 5;
+    const uint32_t nbytes = 4 * L;
+    for (int ct = 0; ct < kMfmaCoeffTiles; ++ct) {
+        const uint32_t c0 = wg_base + (wave * kMfmaCoeffTiles + ct) * 32;
+        if (c0 >= N) break;  // wave-uniform; no barrier follows
+        const uint32_t c = c0 + col;
+        const char* cp = reinterpret_cast<const char*>(src + (size_t)c * L);
+        nc_v4i bf[kMfmaK / 32];
+        uint32_t sa = 0;
+#pragma unroll
+        for (int ks = 0; ks < kMfmaK / 32; ++ks) {
+            const uint32_t off = 32 * ks + 16 * half;
+            nc_v4i x = {0, 0, 0, 0};
+            if (off < nbytes) x = *reinterpret_cast<const nc_v4i*>(cp + off);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sa = __builtin_amdgcn_sad_u8((uint32_t)x[e], 0u, sa);
+            bf[ks] = x ^ (int)0x80808080;
+        }
+        sa += __shfl_xor(sa, 32);
+        const int64_t sa_term = (int64_t)sa * (128ll * 0x01010101ll);
+        for (uint32_t t = 0; t < tiles; ++t) {
+            nc_v16i acc = {};
+            const int8_t* arow = lds_tab + (t * 32 + col) * kMfmaRowStride + 16 * half;
+#pragma unroll
+            for (int ks = 0; ks < kMfmaK / 32; ++ks)
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*reinterpret_cast<const nc_v4i*>(arow + 32 * ks), bf[ks],
+                                                            acc, 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const uint32_t j = t * 8 + 2 * h + half;
+                const PrimeLds cj = lds_prime[j];
+                const int64_t v = (int64_t)acc[4 * h] + ((int64_t)acc[4 * h + 1] << 8) +
+                                  ((int64_t)acc[4 * h + 2] << 16) + ((int64_t)acc[4 * h + 3] << 24) + sa_term +
+                                  cj.g;
+                const uint64_t u = (uint64_t)v;
+                const uint32_t lo = (uint32_t)u;
+                const uint32_t r = (uint32_t)(u >> 32) + __umulhi(lo * cj.mont_neg, cj.p) + (lo != 0);
+                if (j < k) dst[(size_t)j * N + c] = W30::sub_if_ge(r, cj.p);
+            }
+        }
+    }
+}
+
 // ===========================================================================
 // NTT passes on registers.  A pass handles R consecutive index bits
 // [lo, lo + R) of the values a thread holds.  Register e = (eq << R) | ep:
@@ -779,7 +867,7 @@ static hipError_t outer_inv_dispatch(const DevPlan& d, void* ra, uint32_t nprod,
 }
 
 hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void* res_a, uint32_t na,
-                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s) {
+                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma) {
     const dim3 grid(((d.N + 63) / 64) * (na + nb));
     if (width == 62) {
         uint64_t* ra = static_cast<uint64_t*>(res_a);
@@ -789,6 +877,12 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
         else
             decompose62_kernel<kMaxLimbs><<<grid, 256, 0, s>>>(a, ra, na, b, rb, d.w62.place, d.w62.pc, d.N, d.L,
                                                                d.w62.k);
+    } else if (mfma && d.w30.mfma_tab && d.N >= 32) {
+        constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
+        const dim3 g(((d.N + kPerWg - 1) / kPerWg) * (na + nb));
+        decompose30_mfma_kernel<<<g, 256, 0, s>>>(a, static_cast<uint32_t*>(res_a), na, b,
+                                                  static_cast<uint32_t*>(res_b), d.w30.mfma_tab, d.w30.mfma_g,
+                                                  d.w30.pc, d.N, d.L, d.w30.k, d.w30.mfma_tiles);
     } else {
         uint32_t* ra = static_cast<uint32_t*>(res_a);
         uint32_t* rb = static_cast<uint32_t*>(res_b);

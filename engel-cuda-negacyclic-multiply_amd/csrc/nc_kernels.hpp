@@ -31,6 +31,10 @@ struct DevSet30 {
     const CrtConst32* cc = nullptr;
     const uint32_t* miq = nullptr;
     const QConst* qc = nullptr;
+    // MFMA decomposition: [tiles][32][kMfmaK] int8 and G_j (int64), or null
+    const int8_t* mfma_tab = nullptr;
+    const int64_t* mfma_g = nullptr;
+    uint32_t mfma_tiles = 0;
 };
 
 // Device-side view of a plan: dimensions and both prime sets.
@@ -43,8 +47,10 @@ struct DevPlan {
 // Residue buffers are u64 (width 62) or u32 (width 30), layout [p][k][N].
 
 // limbs -> canonical residues; polys a (na of them) then b (nb) in one launch.
+// mfma: use the int8 matrix-core contraction where available (30-bit primes,
+// L % 4 == 0, N >= 32); otherwise, and when false, the VALU kernels.
 hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void* res_a, uint32_t na,
-                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s);
+                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma = true);
 
 // Residue rows of a and b -> residues of the exact negacyclic product, in
 // place in ra (canonical [0, p)); rb is clobbered.  log_blocks is the outer

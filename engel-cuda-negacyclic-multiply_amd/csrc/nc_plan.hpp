@@ -24,6 +24,8 @@ struct HostSet62 {
 };
 
 struct HostSet30 {
+    std::vector<int8_t> mfma_tab;  // [tiles][32][kMfmaK], empty when L % 4 != 0
+    std::vector<int64_t> mfma_g;   // [k]
     std::vector<uint64_t> primes;
     std::vector<Tw32> tw_fwd, tw_inv;
     std::vector<PrimeConst32> pc;
@@ -51,6 +53,7 @@ void self_test(const HostTables& t);
 struct Choice {
     uint32_t width = 62;     // prime width: 62 or 30
     uint32_t log_blocks = 0; // NTT outer split
+    bool mfma = true;        // int8 matrix-core decomposition where available
 };
 
 }  // namespace nc
@@ -81,4 +84,5 @@ struct nc_plan {
     Tuned tune[32];
     bool forced = false;
     nc::Choice forced_choice;
+    int mfma_mode = -1;  // -1: tuned (default on), 0: off, 1: on (nc_plan_set_mfma)
 };

@@ -205,6 +205,31 @@ void build30(HostTables& t, const Big& qb) {
             pl = mulmod(pl, c.c32, p);
         }
     }
+    // int8 MFMA decomposition table: X_u = sum_m a_m t_um (byte u of
+    // T_m = 2^(8m) 2^32 mod p) is computed as D_u = sum_m (a_m - 128)(t_um - 128),
+    // so X_u = D_u + 128 Sa + 128 sum_m t_um - 4L 128^2 (Sa = sum_m a_m), and
+    // sum_u X_u 2^(8u) = sum_u D_u 2^(8u) + 128 Sa 0x01010101 + G_j.
+    if (t.L % 4 == 0) {
+        const uint32_t tiles = (k + 7) / 8, nbytes = 4 * t.L;
+        s.mfma_tab.assign((size_t)tiles * 32 * kMfmaK, 0);
+        s.mfma_g.assign(k, 0);
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint64_t p = s.primes[j];
+            uint64_t place = (1ull << 32) % p;  // 2^(8m) 2^32 mod p, m = 0
+            int64_t col[4] = {0, 0, 0, 0};
+            for (uint32_t m = 0; m < nbytes; ++m) {
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t byte = (uint32_t)(place >> (8 * u)) & 0xffu;
+                    col[u] += byte;
+                    s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m] = (int8_t)((int)byte - 128);
+                }
+                place = mulmod(place, 256, p);
+            }
+            int64_t g = 0;
+            for (int u = 0; u < 4; ++u) g += (128 * col[u] - (int64_t)nbytes * 16384) * ((int64_t)1 << (8 * u));
+            s.mfma_g[j] = g;
+        }
+    }
     std::vector<uint64_t> y;
     crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {

@@ -38,7 +38,7 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [
         ("N", ctypes.c_uint32), ("L", ctypes.c_uint32), ("q_bits", ctypes.c_uint32), ("k62", ctypes.c_uint32),
         ("k30", ctypes.c_uint32), ("width", ctypes.c_uint32), ("log_blocks", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
+        ("decompose_mfma", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
     ]
 
 
@@ -62,6 +62,8 @@ def lib():
         L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
         L.nc_plan_primes.argtypes = [_vp, ctypes.c_uint32, _u64p]
         L.nc_plan_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.nc_plan_set_mfma.argtypes = [_vp, ctypes.c_int]
+        L.nc_plan_get_choice.argtypes = [_vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 3
         L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
         L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
@@ -157,6 +159,18 @@ class Plan:
         """Pin prime width (62 / 30) and NTT split; width 0 unpins."""
         lb = 0xFFFFFFFF if log_blocks is None else int(log_blocks)
         _check(lib().nc_plan_set_tuning(self._h, int(width), lb), "nc_plan_set_tuning")
+        self.info = self.get_info()
+
+    def choice(self, batch: int):
+        """(prime width, NTT split, matrix-core decomposition) used for `batch` products."""
+        w, lb, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().nc_plan_get_choice(self._h, batch, ctypes.byref(w), ctypes.byref(lb), ctypes.byref(m)),
+               "nc_plan_get_choice")
+        return int(w.value), int(lb.value), int(m.value)
+
+    def set_mfma(self, mode: int):
+        """Matrix-core decomposition: 1 on, 0 off, -1 tuned."""
+        _check(lib().nc_plan_set_mfma(self._h, int(mode)), "nc_plan_set_mfma")
         self.info = self.get_info()
 
     def autotune(self, batch: int, stream=None):

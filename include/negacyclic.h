@@ -57,7 +57,7 @@ typedef struct nc_plan_info {
     uint32_t k30;          /* primes in the 30-bit set                      */
     uint32_t width;        /* prime width used for one product per call     */
     uint32_t log_blocks;   /* its NTT outer split (log2 blocks per row)     */
-    uint32_t reserved;
+    uint32_t decompose_mfma; /* 1 if it decomposes on the matrix cores      */
     uint64_t table_bytes;  /* resident device bytes of the plan's tables    */
     uint64_t bytes_per_product; /* device workspace bytes per product       */
 } nc_plan_info;
@@ -84,6 +84,16 @@ int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
  * the autotuned / default choices.  Any accepted setting yields bit-identical
  * output. */
 int nc_plan_set_tuning(nc_plan* plan, uint32_t width, uint32_t log_blocks);
+
+/* The configuration calls with `batch` products use now (tuned, forced or
+ * default): prime width, NTT outer split, matrix-core decomposition (0/1). */
+int nc_plan_get_choice(const nc_plan* plan, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
+                       uint32_t* decompose_mfma);
+
+/* Decomposition on the matrix cores (30-bit primes, L % 4 == 0, N >= 32):
+ * mode 1 always, 0 never (VALU kernels), -1 let nc_plan_autotune decide
+ * (default: on until tuned).  Output bits never depend on it. */
+int nc_plan_set_mfma(nc_plan* plan, int mode);
 
 /* Autotuner: times every (prime width, NTT split) for `batch` products on
  * `stream` (zero inputs; the kernels' work does not depend on the data) and

@@ -66,12 +66,17 @@ def test_stage_residues_62(gpu, N, q868):
     assert np.array_equal(got[1], orc.residues(a[::-1].copy()))
 
 
-def test_stage_residues_30(gpu, q868):
+@pytest.mark.parametrize("mfma", [1, 0])
+def test_stage_residues_30(gpu, q868, mfma):
+    """30-bit residues on the matrix cores (int8 byte split) and on the VALU."""
     N = 1024
-    say("stage residues 30", N)
+    say("stage residues 30 mfma", mfma, N)
     a = ncmul.gen_uniform(22, N, q868)
+    a[5] = O.int_to_limbs(O.limbs_to_int(q868) - 1, 28)  # all-ones bytes where q allows
+    a[6] = 0
     ai = O.poly_to_ints(a)
     with ncmul.Plan(N, q868) as plan:
+        plan.set_mfma(mfma)
         pr = [int(p) for p in plan.primes(30)]
         assert all(p < (1 << 30) and p % (2 * N) == 1 for p in pr)
         d_a = ncmul.to_device_u32(a[None])
@@ -80,6 +85,24 @@ def test_stage_residues_30(gpu, q868):
         got = ncmul.from_device_u32(d_res)[0]
     for j, p in enumerate(pr):
         assert [int(x) for x in got[j]] == [x % p for x in ai], j
+
+
+@pytest.mark.parametrize("N", [32, 512, 16384, 65536])
+def test_residues_30_mfma_equals_valu(gpu, N, q868):
+    say("mfma vs valu residues", N)
+    B = 3
+    a = np.stack([ncmul.gen_uniform(70 + i, N, q868) for i in range(B)])
+    with ncmul.Plan(N, q868) as plan:
+        d_a = ncmul.to_device_u32(a)
+        outs = []
+        for mfma in (1, 0):
+            plan.set_mfma(mfma)
+            d_res = ncmul.empty_device_u32((B, plan.k30, N))
+            plan.stage_residues(30, d_a, d_res, B)
+            outs.append(ncmul.from_device_u32(d_res))
+        assert np.array_equal(outs[0], outs[1])
+        p0 = int(plan.primes(30)[0])
+        assert [int(x) for x in outs[0][1, 0, :8]] == [x % p0 for x in O.poly_to_ints(a[1][:8])]
 
 
 @pytest.mark.parametrize("N", [1024, 16384])
@@ -126,9 +149,11 @@ def test_tuning_invariance(gpu, N, width, q868):
         logn = N.bit_length() - 1
         lo, hi = max(0, logn - 14), min(4, logn - 4)
         for lb in range(lo, hi + 1):
-            say("tuning", N, width, lb)
-            plan.set_tuning(width, lb)
-            assert np.array_equal(plan.multiply(a, b), want), lb
+            for mfma in ((1, 0) if width == 30 else (0,)):
+                say("tuning", N, width, lb, "mfma", mfma)
+                plan.set_mfma(mfma)
+                plan.set_tuning(width, lb)
+                assert np.array_equal(plan.multiply(a, b), want), (lb, mfma)
 
 
 @pytest.mark.parametrize("width", WIDTHS)
