@@ -258,8 +258,8 @@ def main() -> int:
             "workload": f"negacyclic product mod (X^{N}+1, q), q = 2^868-467 (L=28 u32 limbs), "
                         f"{B} products per step, device resident",
             "N": N, "L": L, "batch": B, "k62": batched.k62, "k30": batched.k30,
-            "batched": {"prime_width": w_batch, "ntt_split": s1_batch, "mfma_decompose": batched.choice(B)[2]},
-            "plain": {"prime_width": w_plain, "ntt_split": s1_plain, "mfma_decompose": plain.choice(1)[2]},
+            "batched": {"prime_width": w_batch, "ntt_split": s1_batch, "mfma_paths": batched.choice(B)[2]},
+            "plain": {"prime_width": w_plain, "ntt_split": s1_plain, "mfma_paths": plain.choice(1)[2]},
         },
         "plain": {
             "device_products_per_s": plain_dev_pps,

@@ -13,6 +13,9 @@
 #include "nc_plan.hpp"
 
 using nc::Choice;
+using nc::kMfmaAll;
+using nc::kMfmaCrt;
+using nc::kMfmaDecompose;
 
 namespace {
 
@@ -48,7 +51,18 @@ uint32_t k_of(const nc_plan* p, uint32_t width) {
 // Untuned default: 30-bit primes, and the smallest outer split that still
 // gives about 1024 workgroup-sized blocks (k * batch * 2^s1): fewer, larger
 // blocks cost fewer HBM round trips but must still fill 256 compute units.
-bool mfma_default(const nc_plan* p) { return p->mfma_mode != 0; }
+uint32_t mfma_default(const nc_plan* p) { return p->mfma_mode < 0 ? kMfmaAll : (uint32_t)p->mfma_mode; }
+
+// Matrix-core paths this plan can run (30-bit set).
+uint32_t mfma_available(const nc_plan* p) {
+    uint32_t m = 0;
+    if (p->dev.w30.mfma_tab && p->host.N >= 32) m |= kMfmaDecompose;
+    if (nc::crt_mfma_available(p->dev)) m |= kMfmaCrt;
+    return m;
+}
+
+// Paths a call with choice c actually runs.
+uint32_t mfma_in_use(const nc_plan* p, const Choice& c) { return c.width == 30 ? (c.mfma & mfma_available(p)) : 0u; }
 
 Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
     const uint32_t log_n = p->host.log_n;
@@ -69,7 +83,7 @@ Choice choose(const nc_plan* p, uint32_t batch) {
     if (p->forced) c = p->forced_choice;
     else if (t.tuned) c = t.choice;
     else c = heuristic(p, batch);
-    if (p->mfma_mode >= 0) c.mfma = p->mfma_mode == 1;
+    if (p->mfma_mode >= 0) c.mfma = (uint32_t)p->mfma_mode;
     return c;
 }
 
@@ -100,10 +114,11 @@ int check_width(uint32_t width) {
 
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
-    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, c.mfma),
+    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s,
+                                (c.mfma & kMfmaDecompose) != 0),
            "decompose launch");
     NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
-    NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s), "crt launch");
+    NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
     return NC_OK;
 }
 
@@ -144,6 +159,21 @@ void nc_plan_destroy(nc_plan* p) {
     if (p->h_pin) (void)hipHostFree(p->h_pin);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
+}
+
+int nc_check_tables(uint32_t N, uint32_t L, const uint32_t* q) {
+    if (!q) return fail(NC_ERR_INVALID, "q is null");
+    try {
+        nc::self_test(nc::build_tables(N, L, q));
+    } catch (const std::invalid_argument& e) {
+        return fail(NC_ERR_INVALID, e.what());
+    } catch (const std::runtime_error& e) {
+        const std::string m = e.what();
+        return fail(m.rfind("self test", 0) == 0 ? NC_ERR_SELFTEST : NC_ERR_PRIMES, m);
+    } catch (const std::bad_alloc&) {
+        return fail(NC_ERR_NOMEM, "host tables");
+    }
+    return NC_OK;
 }
 
 int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
@@ -192,6 +222,8 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     const size_t o30_cc = slot(b.cc.size() * sizeof(nc::CrtConst32)), o30_mq = slot(b.miq.size() * 4);
     const size_t o30_qc = slot(sizeof(nc::QConst));
     const size_t o30_mt = slot(b.mfma_tab.size()), o30_mg = slot(b.mfma_g.size() * sizeof(int64_t));
+    const size_t o30_ct = slot(b.crt_tab.size()), o30_cc2 = slot(b.crt_c.size() * sizeof(int64_t));
+    const size_t o30_cm = slot(sizeof(nc::CrtMfmaConst));
     p->table_bytes = off;
     e = hipMalloc(&p->d_tables, off);
     if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
@@ -204,6 +236,8 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         upload(base, o30_pl, b.place),   upload(base, o30_cc, b.cc),      upload(base, o30_mq, b.miq),
         hipMemcpy(base + o30_qc, &b.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
         upload(base, o30_mt, b.mfma_tab), upload(base, o30_mg, b.mfma_g),
+        upload(base, o30_ct, b.crt_tab),  upload(base, o30_cc2, b.crt_c),
+        hipMemcpy(base + o30_cm, &b.crt_mc, sizeof(nc::CrtMfmaConst), hipMemcpyHostToDevice),
     };
     for (hipError_t u : ups)
         if (u != hipSuccess) return cleanup(hip_fail(u, "table upload"));
@@ -232,6 +266,17 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         d.w30.mfma_g = reinterpret_cast<const int64_t*>(base + o30_mg);
         d.w30.mfma_tiles = (uint32_t)(b.mfma_tab.size() / (32 * nc::kMfmaK));
     }
+    if (!b.crt_tab.empty()) {
+        d.w30.crt_tab = reinterpret_cast<const int8_t*>(base + o30_ct);
+        d.w30.crt_c = reinterpret_cast<const int64_t*>(base + o30_cc2);
+        d.w30.crt_mc = reinterpret_cast<const nc::CrtMfmaConst*>(base + o30_cm);
+        d.w30.crt_rt = b.crt_rt;
+        d.w30.crt_kc = b.crt_kc;
+    }
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "device attribute"));
+    d.cus = cus > 0 ? (uint32_t)cus : 1u;
     p->res_bytes = std::max((size_t)d.w62.k * 8, (size_t)d.w30.k * 4) * t.N;
 
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
@@ -257,7 +302,7 @@ int nc_plan_get_info(const nc_plan* p, nc_plan_info* info) {
     info->k30 = k_of(p, 30);
     info->width = c.width;
     info->log_blocks = c.log_blocks;
-    info->decompose_mfma = (c.width == 30 && c.mfma && p->dev.w30.mfma_tab && p->host.N >= 32) ? 1u : 0u;
+    info->mfma_paths = mfma_in_use(p, c);
     info->table_bytes = p->table_bytes;
     info->bytes_per_product = 2ull * p->res_bytes;
     return NC_OK;
@@ -296,20 +341,19 @@ int nc_plan_set_tuning(nc_plan* p, uint32_t width, uint32_t log_blocks) {
 }
 
 int nc_plan_get_choice(const nc_plan* p, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
-                       uint32_t* decompose_mfma) {
+                       uint32_t* mfma_paths) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
     if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
     const Choice c = choose(p, batch);
     if (width) *width = c.width;
     if (log_blocks) *log_blocks = c.log_blocks;
-    if (decompose_mfma)
-        *decompose_mfma = (c.width == 30 && c.mfma && p->dev.w30.mfma_tab && p->host.N >= 32) ? 1u : 0u;
+    if (mfma_paths) *mfma_paths = mfma_in_use(p, c);
     return NC_OK;
 }
 
 int nc_plan_set_mfma(nc_plan* p, int mode) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
-    if (mode < -1 || mode > 1) return fail(NC_ERR_INVALID, "mode must be -1, 0 or 1");
+    if (mode < -1 || mode > (int)kMfmaAll) return fail(NC_ERR_INVALID, "mode must be -1 or a mask of 1 and 2");
     p->mfma_mode = mode;
     return NC_OK;
 }
@@ -334,20 +378,19 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     const uint32_t log_n = p->host.log_n;
     const uint32_t lo = log_n >= 4 ? nc::min_log_blocks(log_n) : 0u;
     const uint32_t hi = log_n >= 4 ? std::max(lo, nc::max_log_blocks(log_n)) : 0u;
-    const bool mfma_avail = p->dev.w30.mfma_tab != nullptr && p->host.N >= 32;
+    const uint32_t avail = mfma_available(p);
     struct Cand {
         uint32_t width;
-        bool mfma;
+        uint32_t mfma;
     };
     std::vector<Cand> cands;
-    for (uint32_t width : {30u, 62u}) {
-        if (width == 30 && mfma_avail && p->mfma_mode < 0) {
-            cands.push_back({30u, true});
-            cands.push_back({30u, false});
-        } else {
-            cands.push_back({width, mfma_default(p)});
-        }
+    if (p->mfma_mode < 0) {  // every distinct subset of the available paths
+        for (uint32_t m = 0; m <= kMfmaAll; ++m)
+            if ((m & avail) == m) cands.push_back({30u, m});
+    } else {
+        cands.push_back({30u, mfma_default(p)});
     }
+    cands.push_back({62u, 0u});
     for (const Cand& cd : cands) {
         for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb) {
             Choice c;
@@ -422,7 +465,7 @@ int nc_stage_residues(nc_plan* p, uint32_t width, const uint32_t* d_poly, void* 
     if (int rc = check_width(width)) return rc;
     if (int rc = check_batch(p, batch)) return rc;
     NC_HIP(nc::launch_decompose(p->dev, (int)width, d_poly, d_res, batch, nullptr, nullptr, 0,
-                                static_cast<hipStream_t>(stream), choose(p, batch).mfma),
+                                static_cast<hipStream_t>(stream), (choose(p, batch).mfma & kMfmaDecompose) != 0),
            "decompose launch");
     return NC_OK;
 }
@@ -438,7 +481,8 @@ int nc_stage_product_residues(nc_plan* p, uint32_t width, const uint32_t* d_a, c
     Choice c = choose(p, batch);
     if (c.width != width) c = heuristic(p, batch, width);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s, c.mfma),
+    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s,
+                                (c.mfma & kMfmaDecompose) != 0),
            "decompose launch");
     NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
     return NC_OK;
@@ -448,7 +492,9 @@ int nc_stage_crt(nc_plan* p, uint32_t width, const void* d_res, uint32_t* d_c, u
     if (!p || !d_res || !d_c) return fail(NC_ERR_INVALID, "null argument");
     if (int rc = check_width(width)) return rc;
     if (int rc = check_batch(p, batch)) return rc;
-    NC_HIP(nc::launch_crt(p->dev, (int)width, d_res, d_c, batch, static_cast<hipStream_t>(stream)), "crt launch");
+    NC_HIP(nc::launch_crt(p->dev, (int)width, d_res, d_c, batch, static_cast<hipStream_t>(stream),
+                          (choose(p, batch).mfma & kMfmaCrt) != 0),
+           "crt launch");
     return NC_OK;
 }
 
@@ -469,7 +515,8 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
         hipError_t e = hipEventRecord(ev[0], s);
         if (e == hipSuccess)
-            e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, c.mfma);
+            e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s,
+                                     (c.mfma & kMfmaDecompose) != 0);
         if (e == hipSuccess) e = hipEventRecord(ev[1], s);
         if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[2], s);
@@ -477,7 +524,7 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
         if (e == hipSuccess) e = hipEventRecord(ev[3], s);
         if (e == hipSuccess) e = nc::launch_ntt_outer_inv(p->dev, w, p->d_res_a, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[4], s);
-        if (e == hipSuccess) e = nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s);
+        if (e == hipSuccess) e = nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0);
         if (e == hipSuccess) e = hipEventRecord(ev[5], s);
         if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
         for (int i = 0; i < 5 && e == hipSuccess; ++i) {

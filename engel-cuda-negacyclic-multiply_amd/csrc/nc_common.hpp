@@ -71,6 +71,27 @@ constexpr int kMfmaK = 128;                        // contraction length (bytes 
 constexpr int kMfmaRowStride = kMfmaK + 16;        // LDS row pitch: conflict-free 16-byte reads
 constexpr int kMfmaTiles = (kMaxPrimes30 + 7) / 8; // 8 primes x 4 byte chunks = 32 rows per tile
 
+// Byte-split int8 MFMA CRT table (30-bit primes).  Contraction index
+// kappa = 4 j + u (prime j, byte u of v_j; chunks of 8 primes = 32 bytes).
+// Output row (lambda, b) sits at 32 (lambda / 8) + 8 ((lambda % 8) / 2) +
+// 4 (lambda % 2) + b, so lane half h of v_mfma_i32_32x32x32_i8 receives whole
+// pseudo-limbs lambda = 8 rt + 2 m + h (m = 0..3) of row tile rt.  With RT row
+// tiles and LQ = min(8 RT - 4, 32): lambda < LQ are the limbs of
+// sum_j v_j ((M / p_j) mod q), lambda = 8 RT - 4, 8 RT - 3 the two limbs of
+// the fraction sum_j v_j floor(2^56 / p_j), lambda = 8 RT - 2, 8 RT - 1 (byte
+// 0) a row of ones that yields the byte sum of the coefficient's v_j.
+constexpr int kCrtMaxKC = (kMaxPrimes30 + 7) / 8;   // contraction chunks (8 primes each)
+constexpr int kCrtRowStride = kCrtMaxKC * 32 + 16;  // LDS row pitch: conflict-free 16-byte reads
+constexpr int kCrtMaxRT = 5;                        // (32 + 4) pseudo-limbs / 8 per row tile
+
+// Montgomery finish of the MFMA CRT (R = 2^96): the table carries
+// (M / p_j) mod q times R, so the limb sum x satisfies x = c R (mod q).
+struct CrtMfmaConst {
+    uint32_t neg_m_r[kMaxLimbs];  // ((-M) mod q) R mod q
+    uint32_t q_inv;               // -q^-1 mod 2^32
+    uint32_t pad[3];
+};
+
 // CRT constants for a 30-bit prime: y = (M/p)^-1 mod p (Shoup pair) and
 // g = floor(2^96 / p) = g_hi 2^64 + g_lo for the fixed-point fraction
 // floor(v 2^64 / p) = floor(v g / 2^32) (at most one below).

@@ -542,8 +542,9 @@ __global__ __launch_bounds__(64) void ntt_tiny_kernel(typename A::T* __restrict_
 }
 
 // ===========================================================================
-// CRT: canonical residues -> L limbs mod q.
-//   v_j   = r_j (M/p_j)^-1 mod p_j                   (Shoup, exact)
+// CRT: CRT-weighted residues -> L limbs mod q.
+//   v_j   = r_j (M/p_j)^-1 mod p_j                   (emitted by the inverse
+//                                                     NTT, canonical)
 //   acc   = sum_j v_j ((M/p_j) mod q)                (LT + 3 limbs)
 //   frac  = sum_j floor(v_j 2^64 / p_j), each estimated from below by at
 //           most one unit (128-bit fixed point)
@@ -670,16 +671,23 @@ __global__ __launch_bounds__(256) void crt62_kernel(const uint64_t* __restrict__
 #pragma unroll
     for (int l = 0; l < NA; ++l) acc[l] = 0;
     uint64_t f_lo = 0, f_hi = 0;
-    for (uint32_t j = 0; j < k; ++j) {
-        const uint64_t p = pcs[j].p;
-        const CrtConst cj = cc[j];
-        const uint64_t v = W62::sub_if_ge(W62::shoup(r[(size_t)j * N], cj.y, cj.y_s, p), p);
-        const uint64_t f = v * cj.mag_hi + __umul64hi(v, cj.mag_lo);
-        f_lo += f;
-        f_hi += (f_lo < f);
-        const uint32_t* m = miq + (size_t)j * kMaxLimbs;
-        mac_limbs<NA>(acc, m, LT, (uint32_t)v, 0);
-        mac_limbs<NA>(acc, m, LT, (uint32_t)(v >> 32), 1);
+    for (uint32_t j0 = 0; j0 < k; j0 += 4) {
+        uint64_t vv[4];  // four loads in flight per trip
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vv[e] = r[(size_t)min(j0 + e, k - 1) * N];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t j = j0 + e;
+            if (j >= k) break;
+            const CrtConst cj = cc[j];
+            const uint64_t v = vv[e];
+            const uint64_t f = v * cj.mag_hi + __umul64hi(v, cj.mag_lo);
+            f_lo += f;
+            f_hi += (f_lo < f);
+            const uint32_t* m = miq + (size_t)j * kMaxLimbs;
+            mac_limbs<NA>(acc, m, LT, (uint32_t)v, 0);
+            mac_limbs<NA>(acc, m, LT, (uint32_t)(v >> 32), 1);
+        }
     }
     crt_finish<LT>(acc, f_lo, f_hi, qc, out + ((size_t)prod * N + i) * L, L);
 }
@@ -703,24 +711,31 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
     for (int l = 0; l <= LT; ++l) col[l] = 0;
     uint64_t f_lo = 0, f_hi = 0;
     uint32_t pending = 0;
-    for (uint32_t j = 0; j < k; ++j) {
-        const PrimeConst32 pj = pcs[j];
-        const CrtConst32 cj = cc[j];
-        const uint32_t v = W30::sub_if_ge(W30::shoup(r[(size_t)j * N], cj.y, cj.y_s, pj.p), pj.p);
-        // floor(v g / 2^32), g = floor(2^96 / p) = g_hi 2^64 + g_lo
-        const uint64_t f = ((uint64_t)(v * cj.g_hi) << 32) + (uint64_t)v * (uint32_t)(cj.g_lo >> 32) +
-                           __umulhi(v, (uint32_t)cj.g_lo);
-        f_lo += f;
-        f_hi += (f_lo < f);
-        const uint32_t* m = miq + (size_t)j * kMaxLimbs;
+    for (uint32_t j0 = 0; j0 < k; j0 += 4) {
+        uint32_t vv[4];  // four loads in flight per trip
 #pragma unroll
-        for (int l = 0; l < LT; ++l) col[l] += (uint64_t)v * m[l];
-        if (++pending == 3) {  // carry: col[l] < 3 2^62 + 2^33 < 2^64
-            pending = 0;
+        for (int e = 0; e < 4; ++e) vv[e] = r[(size_t)min(j0 + e, k - 1) * N];
 #pragma unroll
-            for (int l = 0; l < LT; ++l) {
-                col[l + 1] += col[l] >> 32;
-                col[l] = (uint32_t)col[l];
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t j = j0 + e;
+            if (j >= k) break;
+            const CrtConst32 cj = cc[j];
+            const uint32_t v = vv[e];
+            // floor(v g / 2^32), g = floor(2^96 / p) = g_hi 2^64 + g_lo
+            const uint64_t f = ((uint64_t)(v * cj.g_hi) << 32) + (uint64_t)v * (uint32_t)(cj.g_lo >> 32) +
+                               __umulhi(v, (uint32_t)cj.g_lo);
+            f_lo += f;
+            f_hi += (f_lo < f);
+            const uint32_t* m = miq + (size_t)j * kMaxLimbs;
+#pragma unroll
+            for (int l = 0; l < LT; ++l) col[l] += (uint64_t)v * m[l];
+            if (++pending == 3) {  // carry: col[l] < 3 2^62 + 2^33 < 2^64
+                pending = 0;
+#pragma unroll
+                for (int l = 0; l < LT; ++l) {
+                    col[l + 1] += col[l] >> 32;
+                    col[l] = (uint32_t)col[l];
+                }
             }
         }
     }
@@ -733,6 +748,206 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
         carry = t >> 32;
     }
     crt_finish<LT>(acc, f_lo, f_hi, qc, out + ((size_t)prod * N + i) * L, L);
+}
+
+// x -> x 2^-96 mod q for x < q 2^96 (NA limbs), three word-level Montgomery
+// steps and one conditional subtraction; r receives the LQ limbs of the
+// result (zero beyond L).
+template <int LQ, int NA>
+__device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv,
+                                       uint32_t (&r)[LQ]) {
+    static_assert(NA >= LQ + 4, "room for the three steps");
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+        const uint32_t u = x[st] * q_inv;
+        uint32_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < NA - st; ++l) {
+            const uint64_t t = (l < LQ ? (uint64_t)u * qc->q[l] : 0ull) + x[st + l] + carry;
+            x[st + l] = (uint32_t)t;
+            carry = (uint32_t)(t >> 32);
+        }
+    }
+    // r = x[3 ..] < 2q: subtract q once if r >= q
+    uint32_t d[LQ + 1];
+    uint32_t br = 0;
+#pragma unroll
+    for (int l = 0; l <= LQ; ++l) {
+        const uint64_t t = (uint64_t)x[3 + l] - (l < LQ ? qc->q[l] : 0u) - br;
+        d[l] = (uint32_t)t;
+        br = (uint32_t)(t >> 63);
+    }
+#pragma unroll
+    for (int l = 0; l < LQ; ++l) r[l] = br ? x[3 + l] : d[l];
+}
+
+// 30-bit primes on the matrix cores.  The weighted sum is an int8
+// contraction over kappa = 4 j + u (byte u of v_j): table rows hold the bytes
+// of 2^(8u) ((M/p_j) mod q) 2^96 mod q, of 2^(8u) floor(2^56 / p_j) and a row
+// of ones (layout and offsets: kCrtRowStride and nc_tables.cpp
+// build_crt_mfma).  A wave takes 64 coefficients as two 32-coefficient tiles;
+// the B operand of lane (c, h) is simply the four words v_j of primes
+// 8 kc + 4 h + e of its tile's coefficient c (xor 0x80808080 makes them
+// signed bytes), and the output gives lane half h the pseudo-limbs
+// lambda = 8 rt + 2 m + h.  One v_permlane32_swap per word hands lane h all
+// pseudo-limbs of tile h's coefficient c, and every lane finishes one
+// coefficient: carry the 64-bit limb sums, round the fraction to K, add
+// K ((-M) mod q) 2^96, and a Montgomery finish removes the 2^96.
+// KCT > 0 fixes the number of contraction chunks at compile time (kc_n ==
+// KCT); KCT == 0 takes kc_n <= kCrtMaxKC at run time.  LT > 0 fixes L.
+// A wave's 64 results (64 L contiguous words of out) leave through its own
+// LDS region as 16-byte stores.
+template <int RT, int KCT, int LT>
+__device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res, uint32_t* __restrict__ out,
+                                                const int8_t* __restrict__ tab, const int64_t* __restrict__ corr,
+                                                const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
+                                                uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n,
+                                                uint32_t nprod) {
+    constexpr int LF = 8 * RT - 4;                         // fraction pseudo-limbs LF, LF + 1
+    constexpr int LQ = LF < kMaxLimbs ? LF : kMaxLimbs;  // q limbs (zero beyond L)
+    constexpr int KCM = KCT > 0 ? KCT : kCrtMaxKC;
+    __shared__ int8_t lds_tab[RT * 32 * kCrtRowStride];
+    __shared__ __attribute__((aligned(16))) uint32_t lds_out[4 * 64 * (LT > 0 ? LT : kMaxLimbs)];
+    if (KCT > 0) kc_n = KCT;
+    if (LT > 0) L = LT;
+    const uint32_t kbytes = 32 * kc_n;
+    const uint32_t chunks = kbytes / 16;
+    for (uint32_t w = threadIdx.x; w < RT * 32 * chunks; w += 256) {
+        const uint32_t row = w / chunks, ch = w - row * chunks;
+        *reinterpret_cast<nc_v4i*>(lds_tab + row * kCrtRowStride + ch * 16) =
+            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kbytes + ch * 16);
+    }
+    __syncthreads();
+    const uint32_t N = 1u << log_n;
+    const uint32_t total = nprod << log_n;  // a multiple of 32 (N >= 32)
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 31u, half = lane >> 5;
+    const int64_t sb_off = 128ll * kbytes;
+    const uint32_t q_inv = mc->q_inv;
+    for (uint32_t grp = blockIdx.x; grp * 256u < total; grp += gridDim.x) {
+        const uint32_t g0 = grp * 256u + wave * 64u;
+        if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
+        // B operands of both tiles, every load issued before the first use;
+        // rows j >= k read row k - 1 and are replaced by 0
+        nc_v4i bf[2][KCM];
+#pragma unroll
+        for (int tile = 0; tile < 2; ++tile) {
+            const uint32_t t0 = g0 + 32u * tile;
+            const uint32_t tv = t0 < total ? t0 : g0;  // a missing second tile re-reads the first
+            const uint32_t prod = uniform(tv >> log_n);
+            const uint32_t* base = res + (size_t)prod * k * N;
+            const uint32_t off0 = (tv & (N - 1)) + col;
+#pragma unroll
+            for (int kc = 0; kc < KCM; ++kc) {
+                if ((uint32_t)kc < kc_n) {  // wave-uniform
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const uint32_t j = 8 * kc + 4 * half + e;
+                        const uint32_t row = j < k ? j : k - 1;
+                        bf[tile][kc][e] = (int)base[row * N + off0];
+                    }
+                }
+            }
+        }
+        int32_t dones[2];
+        int32_t elo[RT][4], ehi[RT][4], olo[RT][4], ohi[RT][4];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            nc_v16i acc0 = {}, acc1 = {};
+            const int8_t* arow = lds_tab + (rt * 32 + col) * kCrtRowStride + 16 * half;
+#pragma unroll
+            for (int kc = 0; kc < KCM; ++kc) {
+                if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
+                nc_v4i b0, b1;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const bool live = 8 * kc + 4 * half + e < k;
+                    b0[e] = live ? bf[0][kc][e] ^ (int)0x80808080 : (int)0x80808080;
+                    b1[e] = live ? bf[1][kc][e] ^ (int)0x80808080 : (int)0x80808080;
+                }
+                const nc_v4i a = *reinterpret_cast<const nc_v4i*>(arow + 32 * kc);
+                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b1, acc1, 0, 0, 0);
+            }
+            if (rt == RT - 1) {  // ones row of pseudo-limb LF + 2 + h (before the swap: own tile)
+                dones[0] = acc0[12];
+                dones[1] = acc1[12];
+            }
+            // |D| <= Kdim 2^14 < 2^23: both byte pairs fit int32 exactly.  Lane
+            // (c, 0) keeps tile 0's even pseudo-limbs and receives its odd ones;
+            // lane (c, 1) receives tile 1's even ones and keeps the odd.
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t lo0 = (uint32_t)(acc0[4 * m] + (int32_t)((uint32_t)acc0[4 * m + 1] << 8));
+                const uint32_t hi0 = (uint32_t)(acc0[4 * m + 2] + (int32_t)((uint32_t)acc0[4 * m + 3] << 8));
+                const uint32_t lo1 = (uint32_t)(acc1[4 * m] + (int32_t)((uint32_t)acc1[4 * m + 1] << 8));
+                const uint32_t hi1 = (uint32_t)(acc1[4 * m + 2] + (int32_t)((uint32_t)acc1[4 * m + 3] << 8));
+                const auto sl = __builtin_amdgcn_permlane32_swap(lo0, lo1, false, false);
+                const auto sh = __builtin_amdgcn_permlane32_swap(hi0, hi1, false, false);
+                elo[rt][m] = (int32_t)sl[0];
+                olo[rt][m] = (int32_t)sl[1];
+                ehi[rt][m] = (int32_t)sh[0];
+                ohi[rt][m] = (int32_t)sh[1];
+            }
+        }
+        const int64_t sb = (int64_t)(half ? dones[1] : dones[0]) + sb_off;
+        const int64_t sbterm = 128 * sb * 0x01010101ll;
+        auto limb_sum = [&](int lam) -> int64_t {
+            const int rt = lam / 8, m = (lam % 8) / 2;
+            const int32_t lo = (lam & 1) ? olo[rt][m] : elo[rt][m];
+            const int32_t hi = (lam & 1) ? ohi[rt][m] : ehi[rt][m];
+            return (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sbterm + corr[lam];
+        };
+        constexpr int NA = LQ + 4;
+        uint32_t x[NA];
+        int64_t carry = 0;
+#pragma unroll
+        for (int lam = 0; lam < LQ; ++lam) {
+            const int64_t t = limb_sum(lam) + carry;
+            x[lam] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        x[LQ] = (uint32_t)carry;  // the sum is non-negative and below 2^(32 L + 17)
+#pragma unroll
+        for (int l = LQ + 1; l < NA; ++l) x[l] = 0;
+        // K = round(sum_j v_j / p_j) from F = sum_j v_j floor(2^56 / p_j)
+        const uint64_t F = (uint64_t)limb_sum(LF) + ((uint64_t)limb_sum(LF + 1) << 32);
+        const uint32_t K = (uint32_t)(F >> 56) + (uint32_t)((F >> 55) & 1u);
+        mac_limbs<NA>(x, mc->neg_m_r, LQ, K, 0);
+        uint32_t r[LQ];
+        redc96<LQ, NA>(x, qc, q_inv, r);
+        // lane (c, h) holds coefficient g0 + 32 h + c = g0 + lane
+        uint32_t* st = lds_out + wave * 64u * L;
+#pragma unroll
+        for (int l = 0; l < LQ; ++l)
+            if ((uint32_t)l < L) st[lane * L + l] = r[l];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t words = (total - g0 < 64u ? total - g0 : 64u) * L;  // a multiple of 4 (32 | count)
+        uint32_t* dst = out + (size_t)g0 * L;
+        for (uint32_t w = lane * 4u; w < words; w += 256u)
+            *reinterpret_cast<nc_v4i*>(dst + w) = *reinterpret_cast<const nc_v4i*>(st + w);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// The fixed-shape instantiation fits two waves per SIMD without spilling;
+// the run-time-shaped one is left to the compiler's register budget.
+template <int RT, int KCT, int LT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void crt30_mfma_kernel(
+    const uint32_t* __restrict__ res, uint32_t* __restrict__ out, const int8_t* __restrict__ tab,
+    const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
+    uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod) {
+    crt30_mfma_body<RT, KCT, LT>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod);
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
+    const uint32_t* __restrict__ res, uint32_t* __restrict__ out, const int8_t* __restrict__ tab,
+    const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
+    uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod) {
+    crt30_mfma_body<RT, 0, 0>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod);
 }
 
 // ===========================================================================
@@ -924,9 +1139,42 @@ hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, u
     return launch_ntt_outer_inv(d, width, ra, nprod, log_blocks, s);
 }
 
-hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s) {
+bool crt_mfma_available(const DevPlan& d) { return d.w30.crt_tab != nullptr && d.N >= 32; }
+
+template <int RT, int KCT = 0, int LT = 0>
+static void launch_crt_mfma(const DevPlan& d, const uint32_t* res, uint32_t* out, uint32_t nprod, hipStream_t s) {
+    const uint32_t groups = ((nprod << d.log_n) + 255) / 256;
+    const uint32_t lds = RT * 32u * kCrtRowStride + 4u * 64u * 4u * (LT > 0 ? LT : kMaxLimbs);
+    const uint32_t per_cu = (64u * 1024u * 5u / 2u) / lds;  // workgroups whose LDS fits 160 KiB
+    const uint32_t cap = d.cus * (per_cu < 2u ? per_cu : 2u);  // two waves per SIMD at most
+    const uint32_t grid = groups < cap ? groups : cap;
+    if constexpr (KCT > 0)
+        crt30_mfma_kernel<RT, KCT, LT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
+                                                         d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod);
+    else
+        crt30_mfma_generic_kernel<RT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc,
+                                                            d.w30.qc, d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod);
+}
+
+hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s,
+                      bool mfma) {
     const uint32_t total = nprod << d.log_n;
     const dim3 grid((total + 255) / 256);
+    if (width == 30 && mfma && crt_mfma_available(d)) {
+        const uint32_t* r = static_cast<const uint32_t*>(res);
+        switch (d.w30.crt_rt) {
+            case 1: launch_crt_mfma<1>(d, r, out, nprod, s); break;
+            case 2: launch_crt_mfma<2>(d, r, out, nprod, s); break;
+            case 3: launch_crt_mfma<3>(d, r, out, nprod, s); break;
+            case 4:
+                if (d.w30.crt_kc == 8 && d.L == 28) launch_crt_mfma<4, 8, 28>(d, r, out, nprod, s);  // 57..64 primes
+                else launch_crt_mfma<4>(d, r, out, nprod, s);
+                break;
+            case 5: launch_crt_mfma<5>(d, r, out, nprod, s); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (width == 62) {
         const uint64_t* r = static_cast<const uint64_t*>(res);
         if (d.L == 28)

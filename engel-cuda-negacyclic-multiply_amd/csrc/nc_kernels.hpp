@@ -35,11 +35,18 @@ struct DevSet30 {
     const int8_t* mfma_tab = nullptr;
     const int64_t* mfma_g = nullptr;
     uint32_t mfma_tiles = 0;
+    // MFMA CRT: [crt_rt * 32][crt_kc * 32] int8 table and [8 crt_rt]
+    // corrections, or null
+    const int8_t* crt_tab = nullptr;
+    const int64_t* crt_c = nullptr;
+    const CrtMfmaConst* crt_mc = nullptr;
+    uint32_t crt_rt = 0, crt_kc = 0;
 };
 
 // Device-side view of a plan: dimensions and both prime sets.
 struct DevPlan {
     uint32_t N = 0, L = 0, log_n = 0;
+    uint32_t cus = 256;  // compute units of the device
     DevSet62 w62;
     DevSet30 w30;
 };
@@ -52,8 +59,10 @@ struct DevPlan {
 hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void* res_a, uint32_t na,
                             const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma = true);
 
-// Residue rows of a and b -> residues of the exact negacyclic product, in
-// place in ra (canonical [0, p)); rb is clobbered.  log_blocks is the outer
+// Residue rows of a and b -> CRT-weighted residues of the exact negacyclic
+// product, v_j = (c mod p_j) (M / p_j)^-1 mod p_j (the inverse transform's
+// merged N^-1 constant carries the CRT weight), in place in ra (canonical
+// [0, p)); rb is clobbered.  log_blocks is the outer
 // split (0 = the whole row in one workgroup).
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
                               hipStream_t s);
@@ -66,8 +75,12 @@ hipError_t launch_ntt_inner(const DevPlan& d, int width, void* ra, const void* r
 hipError_t launch_ntt_outer_inv(const DevPlan& d, int width, void* ra, uint32_t nprod, uint32_t log_blocks,
                                 hipStream_t s);
 
-// canonical residues -> L limbs mod q.
-hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s);
+// CRT-weighted residues v_j = (c mod p_j) (M / p_j)^-1 mod p_j, canonical,
+// -> L limbs mod q.  mfma: the int8 matrix-core kernel where available
+// (30-bit primes, N >= 32, table present); otherwise the VALU kernels.
+bool crt_mfma_available(const DevPlan& d);
+hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s,
+                      bool mfma = true);
 
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);

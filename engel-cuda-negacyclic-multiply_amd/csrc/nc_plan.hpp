@@ -26,6 +26,13 @@ struct HostSet62 {
 struct HostSet30 {
     std::vector<int8_t> mfma_tab;  // [tiles][32][kMfmaK], empty when L % 4 != 0
     std::vector<int64_t> mfma_g;   // [k]
+    // int8 MFMA CRT: [crt_rt * 32][crt_kc * 32] table (stored offset by -128)
+    // and the per-pseudo-limb corrections [8 crt_rt]; empty when the
+    // modulus is too narrow for its limb count (bits(q) + 54 < 32 L)
+    std::vector<int8_t> crt_tab;
+    std::vector<int64_t> crt_c;
+    CrtMfmaConst crt_mc{};
+    uint32_t crt_rt = 0, crt_kc = 0;
     std::vector<uint64_t> primes;
     std::vector<Tw32> tw_fwd, tw_inv;
     std::vector<PrimeConst32> pc;
@@ -50,10 +57,13 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q);
 void self_test(const HostTables& t);
 
 // Kernel configuration of one call.
+// Matrix-core paths of the 30-bit set (bitmask).
+constexpr uint32_t kMfmaDecompose = 1u, kMfmaCrt = 2u, kMfmaAll = 3u;
+
 struct Choice {
-    uint32_t width = 62;     // prime width: 62 or 30
-    uint32_t log_blocks = 0; // NTT outer split
-    bool mfma = true;        // int8 matrix-core decomposition where available
+    uint32_t width = 62;        // prime width: 62 or 30
+    uint32_t log_blocks = 0;    // NTT outer split
+    uint32_t mfma = kMfmaAll;   // int8 matrix-core paths where available
 };
 
 }  // namespace nc
@@ -84,5 +94,5 @@ struct nc_plan {
     Tuned tune[32];
     bool forced = false;
     nc::Choice forced_choice;
-    int mfma_mode = -1;  // -1: tuned (default on), 0: off, 1: on (nc_plan_set_mfma)
+    int mfma_mode = -1;  // -1: tuned (default all on), else a kMfma* mask (nc_plan_set_mfma)
 };

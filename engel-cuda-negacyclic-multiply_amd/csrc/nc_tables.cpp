@@ -1,6 +1,7 @@
 // Plan tables: the prime sets and every per-prime / per-q constant the
 // kernels read.  Runs once per plan on the host (the reference does the same
 // work in fherma_init, solve.cu:1174-1398, outside the timed call).
+#include <algorithm>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -137,9 +138,16 @@ void build62(HostTables& t, const Big& qb) {
     s.pc.resize(k);
     s.place.assign((size_t)k * kMaxLimbs, 0);
     s.cc.resize(k);
+    // y_j = (M / p_j)^-1 mod p_j rides on the inverse transform's merged N^-1
+    // constants, so the transform emits the CRT-weighted residue
+    // v_j = (c mod p_j) y_j mod p_j that the CRT consumes.
+    std::vector<uint64_t> y;
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
         RootTables r = roots(p, N, t.log_n, 64);
+        r.nr = mulmod(r.nr, y[j], p);
+        r.wr = mulmod(r.wr, y[j], p);
         for (uint32_t x = 0; x < N; ++x) {
             s.tw_fwd[(size_t)j * N + x] = Tw{r.fwd[x], x ? shoup_w(r.fwd[x], p, 64) : 0};
             s.tw_inv[(size_t)j * N + x] = Tw{r.inv[x], x ? shoup_w(r.inv[x], p, 64) : 0};
@@ -160,8 +168,6 @@ void build62(HostTables& t, const Big& qb) {
             pl = mulmod(pl, two32, p);
         }
     }
-    std::vector<uint64_t> y;
-    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
         CrtConst& cc = s.cc[j];
@@ -171,6 +177,82 @@ void build62(HostTables& t, const Big& qb) {
         cc.mag_hi = (uint64_t)(mag >> 64);
         cc.mag_lo = (uint64_t)mag;
     }
+}
+
+// int8 MFMA CRT table.  With kappa = 4 j + u and raw bytes A[row][kappa]:
+//   q-limb rows (lambda < LQ, byte b): byte 4 lambda + b of
+//       m_ju = (2^(8u) ((M / p_j) mod q) R) mod q, R = 2^96, so that
+//       sum_{lambda, b} 2^(32 lambda + 8 b) X[lambda, b] = sum_ju v_ju m_ju
+//       = R sum_j v_j ((M / p_j) mod q)  (mod q),  v_ju byte u of v_j
+//       (the kernel's Montgomery finish removes R);
+//   fraction rows (lambda = LF + e): byte 4 e + b of 2^(8u) floor(2^56 / p_j),
+//       so that the two pseudo-limbs hold F = sum_j v_j floor(2^56 / p_j)
+//       exactly (F < 2^62);
+//   ones rows (lambda = LF + 2, LF + 3, byte 0): stored value 1.
+// The kernel contracts A - 128 with B - 128 (B = bytes of v, xor 0x80); with
+// SA = row sums of A and SB = byte sum of the coefficient's v,
+//   X = D + 128 SA + 128 SB - 128^2 Kdim,
+// and D of a ones row is SB - 128 Kdim.  Per pseudo-limb the kernel forms
+// Y = sum_b 2^(8b) D_b + 128 SB 0x01010101 + C_lambda = sum_b 2^(8b) X_b
+// exactly, with C_lambda = sum_b 2^(8b) (128 SA - 128^2 Kdim): the limb sums
+// then carry into the exact non-negative sum_t 2^(8t) X_t < 2^(32 L + 17).
+void build_crt_mfma(HostTables& t, const Big& qb) {
+    HostSet30& s = t.s30;
+    const uint32_t L = t.L, k = (uint32_t)s.primes.size();
+    // sum_t 2^(8t) X_t < 2^(32 L + 17) must stay below the Barrett range 2^(bits(q) + 71)
+    if (t.q_bits + 54 < 32 * L) return;
+    const uint32_t rt = (L + 4 + 7) / 8, kc = (k + 7) / 8;
+    const uint32_t lf = 8 * rt - 4, lq = std::min(lf, (uint32_t)kMaxLimbs);
+    const uint32_t kbytes = 32 * kc;
+    s.crt_rt = rt;
+    s.crt_kc = kc;
+    s.crt_tab.assign((size_t)rt * 32 * kbytes, 0);
+    s.crt_c.assign((size_t)8 * rt, 0);
+    auto row_of = [](uint32_t lam, uint32_t b) { return 32 * (lam / 8) + 8 * ((lam % 8) / 2) + 4 * (lam % 2) + b; };
+    std::vector<int64_t> sa((size_t)rt * 32, 0);
+    auto put = [&](uint32_t lam, uint32_t b, uint32_t kappa, uint32_t raw) {
+        const uint32_t row = row_of(lam, b);
+        s.crt_tab[(size_t)row * kbytes + kappa] = (int8_t)((int)raw - 128);
+        sa[row] += raw;
+    };
+    for (uint32_t j = 0; j < 8 * kc; ++j) {
+        Big m(L + 4);
+        uint64_t g = 0;
+        if (j < k) {
+            for (uint32_t l = 0; l < L; ++l) m.v[l + 3] = s.miq[(size_t)j * kMaxLimbs + l];  // miq_j R
+            m = mod(m, qb);
+            m.v.resize(L + 1, 0u);
+            g = (uint64_t)((((u128)1) << 56) / s.primes[j]);
+        }
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t kappa = 4 * j + u;
+            Big mr = j < k ? mod(m, qb) : Big(L);  // m_ju = 2^(8u) miq_j mod q
+            for (uint32_t lam = 0; lam < lq; ++lam)
+                for (uint32_t b = 0; b < 4; ++b) {
+                    const uint32_t byte = lam < L ? (mr.v[lam] >> (8 * b)) & 0xffu : 0u;
+                    put(lam, b, kappa, byte);
+                }
+            const uint64_t gu = g << (8 * u);  // < 2^51
+            for (uint32_t e = 0; e < 2; ++e)
+                for (uint32_t b = 0; b < 4; ++b) put(lf + e, b, kappa, (uint32_t)(gu >> (8 * (4 * e + b))) & 0xffu);
+            for (uint32_t e = 2; e < 4; ++e) s.crt_tab[(size_t)row_of(lf + e, 0) * kbytes + kappa] = 1;
+            if (j < k) m.mul_u64(256);
+        }
+    }
+    const int64_t off = 128ll * 128ll * kbytes;  // 128^2 Kdim
+    for (uint32_t lam = 0; lam < 8 * rt; ++lam) {  // exact: sum_b 2^(8b) (128 SA - 128^2 Kdim), |.| < 2^48
+        if (lam >= lq && lam != lf && lam != lf + 1) continue;
+        int64_t c = 0;
+        for (uint32_t b = 0; b < 4; ++b) c += (128 * sa[row_of(lam, b)] - off) * ((int64_t)1 << (8 * b));
+        s.crt_c[lam] = c;
+    }
+    // Montgomery finish constants: ((-M) mod q) R mod q and -q^-1 mod 2^32
+    Big nm(L + 3);
+    for (uint32_t l = 0; l < L; ++l) nm.v[l + 3] = s.qc.neg_m[l];
+    nm = mod(nm, qb);
+    std::memset(&s.crt_mc, 0, sizeof s.crt_mc);
+    for (uint32_t l = 0; l < L; ++l) s.crt_mc.neg_m_r[l] = nm.v[l];
+    s.crt_mc.q_inv = (uint32_t)neg_inv(qb.v[0], 32);
 }
 
 void build30(HostTables& t, const Big& qb) {
@@ -183,9 +265,13 @@ void build30(HostTables& t, const Big& qb) {
     s.pc.resize(k);
     s.place.assign((size_t)k * kMaxLimbs, 0);
     s.cc.resize(k);
+    std::vector<uint64_t> y;  // CRT weight, merged as in build62
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
         RootTables r = roots(p, N, t.log_n, 32);
+        r.nr = mulmod(r.nr, y[j], p);
+        r.wr = mulmod(r.wr, y[j], p);
         for (uint32_t x = 0; x < N; ++x) {
             s.tw_fwd[(size_t)j * N + x] = Tw32{(uint32_t)r.fwd[x], x ? (uint32_t)shoup_w(r.fwd[x], p, 32) : 0u};
             s.tw_inv[(size_t)j * N + x] = Tw32{(uint32_t)r.inv[x], x ? (uint32_t)shoup_w(r.inv[x], p, 32) : 0u};
@@ -230,8 +316,7 @@ void build30(HostTables& t, const Big& qb) {
             s.mfma_g[j] = g;
         }
     }
-    std::vector<uint64_t> y;
-    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
+    build_crt_mfma(t, qb);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
         CrtConst32& cc = s.cc[j];
@@ -265,6 +350,85 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     build30(t, qb);
     return t;
 }
+
+namespace {
+
+// Host emulation of the int8 MFMA CRT contraction (crt30_mfma_kernel) on
+// extreme and random CRT-weighted residues: the q limbs must agree with
+// sum_j v_j ((M / p_j) mod q) mod q, empty limbs must be exactly zero and
+// the fraction limbs must equal sum_j v_j floor(2^56 / p_j).
+void check_crt_mfma(const HostTables& t, std::mt19937_64& rng) {
+    const HostSet30& s = t.s30;
+    const uint32_t L = t.L, k = (uint32_t)s.primes.size();
+    const uint32_t rt = s.crt_rt, kbytes = 32 * s.crt_kc;
+    const uint32_t lf = 8 * rt - 4, lq = std::min(lf, (uint32_t)kMaxLimbs);
+    Big qb(L);
+    for (uint32_t l = 0; l < L; ++l) qb.v[l] = s.qc.q[l];
+    auto row_of = [](uint32_t lam, uint32_t b) { return 32 * (lam / 8) + 8 * ((lam % 8) / 2) + 4 * (lam % 2) + b; };
+    for (int trial = 0; trial < 4; ++trial) {
+        std::vector<uint32_t> v(8 * s.crt_kc, 0u);
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t p = (uint32_t)s.primes[j];
+            v[j] = trial == 0 ? p - 1 : trial == 1 ? 0u : (uint32_t)(rng() % p);
+        }
+        std::vector<int> bq(kbytes);
+        int64_t sb = 0;
+        for (uint32_t kap = 0; kap < kbytes; ++kap) {
+            const uint32_t byte = (v[kap / 4] >> (8 * (kap % 4))) & 0xffu;
+            bq[kap] = (int)byte - 128;
+            sb += byte;
+        }
+        auto dot = [&](uint32_t row) {
+            int64_t d = 0;
+            for (uint32_t kap = 0; kap < kbytes; ++kap) d += (int64_t)s.crt_tab[(size_t)row * kbytes + kap] * bq[kap];
+            return d;
+        };
+        if (dot(row_of(lf + 2, 0)) != sb - 128 * (int64_t)kbytes || dot(row_of(lf + 3, 0)) != sb - 128 * (int64_t)kbytes)
+            throw std::runtime_error("self test: CRT table ones rows");
+        auto Y = [&](uint32_t lam) {
+            int64_t y = 0;
+            for (uint32_t b = 0; b < 4; ++b) y += dot(row_of(lam, b)) * ((int64_t)1 << (8 * b));
+            return y + 128 * sb * 0x01010101ll + s.crt_c[lam];
+        };
+        std::vector<uint32_t> acc(lq + 2, 0u);
+        int64_t carry = 0;
+        for (uint32_t lam = 0; lam < lq; ++lam) {
+            const int64_t y = Y(lam);
+            if (lam >= L && y != 0) throw std::runtime_error("self test: CRT table empty limb");
+            const int64_t x = y + carry;
+            acc[lam] = (uint32_t)x;
+            carry = x >> 32;
+        }
+        if (carry < 0) throw std::runtime_error("self test: CRT table sign");
+        acc[lq] = (uint32_t)carry;
+        acc[lq + 1] = (uint32_t)(carry >> 32);
+        const int64_t f0 = Y(lf), f1 = Y(lf + 1);
+        u128 fwant = 0;
+        for (uint32_t j = 0; j < k; ++j) fwant += (u128)v[j] * (uint64_t)((((u128)1) << 56) / s.primes[j]);
+        if (f0 < 0 || f1 < 0 || (u128)(uint64_t)f0 + ((u128)(uint64_t)f1 << 32) != fwant)
+            throw std::runtime_error("self test: CRT table fraction");
+        Big want(L + 3);
+        for (uint32_t j = 0; j < k; ++j) {
+            Big term(L + 3);
+            for (uint32_t l = 0; l < L; ++l) term.v[l] = s.miq[(size_t)j * kMaxLimbs + l];
+            term.mul_u64(v[j]);
+            uint64_t c = 0;
+            for (uint32_t l = 0; l < L + 3; ++l) {
+                const uint64_t x = (uint64_t)want.v[l] + term.v[l] + c;
+                want.v[l] = (uint32_t)x;
+                c = x >> 32;
+            }
+        }
+        Big wr(L + 6);  // want R
+        for (uint32_t l = 0; l < L + 3; ++l) wr.v[l + 3] = want.v[l];
+        Big got(lq + 2);
+        got.v = acc;
+        if (cmp(mod(got, qb), mod(wr, qb)) != 0) throw std::runtime_error("self test: CRT table sum");
+        if (got.bits() > (int)t.q_bits + 71) throw std::runtime_error("self test: CRT table range");
+    }
+}
+
+}  // namespace
 
 // Known-answer checks of both prime sets, run at plan creation (the
 // analogue of solve.cu:1151-1166, 1285-1314, 1370-1415 for this formulation).
@@ -325,6 +489,7 @@ void self_test(const HostTables& t) {
         for (int l = (int)t.L - 1; l >= 0; --l) horner = (uint64_t)((((u128)horner) << 32 | limbs[l]) % p);
         if (redc != horner) throw std::runtime_error("self test: digit sum does not match the modulus");
     }
+    if (!t.s30.crt_tab.empty()) check_crt_mfma(t, rng);
 }
 
 }  // namespace nc

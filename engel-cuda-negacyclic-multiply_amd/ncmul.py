@@ -23,6 +23,10 @@ STATUS = {
     -5: "NC_ERR_PRIMES", -6: "NC_ERR_SELFTEST", -7: "NC_ERR_CAPACITY", -8: "NC_ERR_STATE",
 }
 
+MFMA_DECOMPOSE = 1  # int8 matrix-core paths of the 30-bit prime set (nc_plan_set_mfma mask)
+MFMA_CRT = 2
+MFMA_ALL = 3
+
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -38,7 +42,7 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [
         ("N", ctypes.c_uint32), ("L", ctypes.c_uint32), ("q_bits", ctypes.c_uint32), ("k62", ctypes.c_uint32),
         ("k30", ctypes.c_uint32), ("width", ctypes.c_uint32), ("log_blocks", ctypes.c_uint32),
-        ("decompose_mfma", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
+        ("mfma_paths", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64), ("bytes_per_product", ctypes.c_uint64),
     ]
 
 
@@ -57,6 +61,7 @@ def lib():
         L.nc_last_error.restype = ctypes.c_char_p
         L.nc_last_error.argtypes = []
         L.nc_plan_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)]
+        L.nc_check_tables.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.nc_plan_destroy.argtypes = [_vp]
         L.nc_plan_destroy.restype = None
         L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
@@ -95,6 +100,12 @@ def gen_uniform(seed: int, N: int, q: np.ndarray) -> np.ndarray:
     out = np.empty((N, q.shape[0]), dtype=np.uint32)
     _check(lib().nc_gen_uniform(ctypes.c_uint64(seed), N, q.shape[0], _p32(q), _p32(out)), "nc_gen_uniform")
     return out
+
+
+def check_tables(N: int, q: np.ndarray):
+    """Host tables and init self tests of (N, q) without a device (raises NcError)."""
+    q = np.ascontiguousarray(q, dtype=np.uint32)
+    _check(lib().nc_check_tables(int(N), q.shape[0], _p32(q)), "nc_check_tables")
 
 
 def measure_mul32_peak() -> float:
@@ -162,14 +173,14 @@ class Plan:
         self.info = self.get_info()
 
     def choice(self, batch: int):
-        """(prime width, NTT split, matrix-core decomposition) used for `batch` products."""
+        """(prime width, NTT split, matrix-core paths mask) used for `batch` products."""
         w, lb, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().nc_plan_get_choice(self._h, batch, ctypes.byref(w), ctypes.byref(lb), ctypes.byref(m)),
                "nc_plan_get_choice")
         return int(w.value), int(lb.value), int(m.value)
 
     def set_mfma(self, mode: int):
-        """Matrix-core decomposition: 1 on, 0 off, -1 tuned."""
+        """Matrix-core paths: a mask of MFMA_DECOMPOSE | MFMA_CRT (0: VALU only), or -1 tuned."""
         _check(lib().nc_plan_set_mfma(self._h, int(mode)), "nc_plan_set_mfma")
         self.info = self.get_info()
 

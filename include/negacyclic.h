@@ -57,7 +57,7 @@ typedef struct nc_plan_info {
     uint32_t k30;          /* primes in the 30-bit set                      */
     uint32_t width;        /* prime width used for one product per call     */
     uint32_t log_blocks;   /* its NTT outer split (log2 blocks per row)     */
-    uint32_t decompose_mfma; /* 1 if it decomposes on the matrix cores      */
+    uint32_t mfma_paths;   /* matrix-core paths it runs (NC_MFMA_* mask)    */
     uint64_t table_bytes;  /* resident device bytes of the plan's tables    */
     uint64_t bytes_per_product; /* device workspace bytes per product       */
 } nc_plan_info;
@@ -75,6 +75,9 @@ const char* nc_last_error(void);
  * upload.  nc_plan_destroy replaces fherma_free (solve.cu:1530-1567) and
  * is null safe. */
 int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out);
+/* Builds the host tables of (N, L, q) and runs the known-answer self tests
+ * without touching a device (the host half of nc_plan_create). */
+int nc_check_tables(uint32_t N, uint32_t L, const uint32_t* q);
 void nc_plan_destroy(nc_plan* plan);
 int nc_plan_get_info(const nc_plan* plan, nc_plan_info* info);
 /* The primes of one set (width 62 or 30), host copy; out holds k values. */
@@ -86,13 +89,20 @@ int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
 int nc_plan_set_tuning(nc_plan* plan, uint32_t width, uint32_t log_blocks);
 
 /* The configuration calls with `batch` products use now (tuned, forced or
- * default): prime width, NTT outer split, matrix-core decomposition (0/1). */
+ * default): prime width, NTT outer split, matrix-core paths (NC_MFMA_*). */
 int nc_plan_get_choice(const nc_plan* plan, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
-                       uint32_t* decompose_mfma);
+                       uint32_t* mfma_paths);
 
-/* Decomposition on the matrix cores (30-bit primes, L % 4 == 0, N >= 32):
- * mode 1 always, 0 never (VALU kernels), -1 let nc_plan_autotune decide
- * (default: on until tuned).  Output bits never depend on it. */
+/* Matrix-core (int8 MFMA) paths of the 30-bit prime set:
+ *   NC_MFMA_DECOMPOSE  limbs -> residues (L % 4 == 0, N >= 32)
+ *   NC_MFMA_CRT        weighted residues -> limbs (N >= 32,
+ *                      bits(q) + 54 >= 32 L)
+ * mode: a mask of these (0 = VALU kernels only), or -1 to let
+ * nc_plan_autotune decide (default: all on until tuned).  A path that is not
+ * available for the plan's shape runs on the VALU.  Output bits never
+ * depend on it. */
+#define NC_MFMA_DECOMPOSE 1
+#define NC_MFMA_CRT 2
 int nc_plan_set_mfma(nc_plan* plan, int mode);
 
 /* Autotuner: times every (prime width, NTT split) for `batch` products on
@@ -123,10 +133,15 @@ int nc_multiply_device(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, 
  *  nc_stage_residues:          res[p][j][i] = poly_p[i] mod p_j (canonical)
  *                              (the value solve.cu:533-570 computes before
  *                              its Montgomery lift and twist)
- *  nc_stage_product_residues:  res[p][j][i] = c_int[i] mod p_j, c_int the
- *                              exact integer negacyclic product (canonical)
+ *  nc_stage_product_residues:  res[p][j][i] = (c_int[i] mod p_j) y_j mod p_j,
+ *                              c_int the exact integer negacyclic product and
+ *                              y_j = (M / p_j)^-1 mod p_j, M the product of
+ *                              the set's primes (canonical; the inverse
+ *                              transform carries the CRT weight y_j that
+ *                              solve.cu:768-776 applies in the CRT)
  *                              (solve.cu:1500-1514 up to untwist)
- *  nc_stage_crt:               d_c from canonical residues (solve.cu:752-836) */
+ *  nc_stage_crt:               d_c from such weighted residues
+ *                              (solve.cu:752-836) */
 int nc_stage_residues(nc_plan* plan, uint32_t width, const uint32_t* d_poly, void* d_res, uint32_t batch,
                       void* stream);
 int nc_stage_product_residues(nc_plan* plan, uint32_t width, const uint32_t* d_a, const uint32_t* d_b, void* d_res,

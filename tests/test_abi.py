@@ -88,3 +88,25 @@ def test_generator_matches_oracle_and_golden(name):
         if case["modulus"] == name and case["N"] <= 16384:
             a = ncmul.gen_uniform(2 * case["seed"], case["N"], q)
             assert hashlib.sha256(a.astype("<u4").tobytes()).hexdigest() == case["sha256_a"]
+
+
+@pytest.mark.parametrize("N,L,bits", [
+    (16384, 28, 868), (1024, 7, 200), (64, 1, 13), (64, 4, 120), (64, 13, 414), (32, 27, 855), (64, 32, 1017),
+    (2, 28, 868), (65536, 28, 868),
+])
+def test_check_tables_host_self_tests(N, L, bits):
+    """Host tables of both prime sets and the init known-answer tests,
+    including the host emulation of the int8 matrix-core CRT contraction,
+    with no device."""
+    import random
+    r = random.Random(N * 131 + L)
+    qi = (1 << 868) - 467 if bits == 868 else (1 << (bits - 1)) | r.getrandbits(bits - 1) | 1
+    ncmul.check_tables(N, O.int_to_limbs(qi, L))
+
+
+def test_check_tables_rejects():
+    with pytest.raises(ncmul.NcError) as e:
+        ncmul.check_tables(1000, modulus("q868"))
+    assert e.value.status == -1
+    with pytest.raises(ncmul.NcError):
+        ncmul.check_tables(1024, np.full(32, 0xFFFFFFFF, dtype=np.uint32))

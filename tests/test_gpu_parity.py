@@ -32,6 +32,29 @@ def say(*a):
     print(*a, flush=True)
 
 
+def crt_weights(primes):
+    """y_j = (M / p_j)^-1 mod p_j: the weight the inverse transform applies."""
+    M = 1
+    for p in primes:
+        M *= p
+    return [pow(M // p, -1, p) for p in primes]
+
+
+def crt_exact(v_cols, primes, qi):
+    """c mod q from weighted residues v_j (one list per coefficient): the
+    centred value of sum_j v_j M / p_j mod M, reduced mod q."""
+    M = 1
+    for p in primes:
+        M *= p
+    out = []
+    for v in v_cols:
+        x = sum(vj * (M // p) for vj, p in zip(v, primes)) % M
+        if x > M // 2:
+            x -= M
+        out.append(x % qi)
+    return out
+
+
 CASES = load_golden()["cases"]
 
 
@@ -110,7 +133,10 @@ def test_stage_product_residues_and_crt_62(gpu, N, q868):
     say("stage product residues 62", N)
     a, b = ncmul.gen_uniform(31, N, q868), ncmul.gen_uniform(32, N, q868)
     orc = O.Plan(N, q868)
-    want = orc.product_residues(a, b)
+    pres = orc.product_residues(a, b)
+    pr = [int(p) for p in orc.primes()]
+    y = crt_weights(pr)
+    want = np.array([[int(r) * y[j] % pr[j] for r in pres[j]] for j in range(len(pr))], dtype=np.uint64)
     with ncmul.Plan(N, q868) as plan:
         d_a, d_b = ncmul.to_device_u32(a[None]), ncmul.to_device_u32(b[None])
         d_res = ncmul.empty_device_u64((1, plan.k62, N))
@@ -118,18 +144,21 @@ def test_stage_product_residues_and_crt_62(gpu, N, q868):
         assert np.array_equal(ncmul.from_device_u64(d_res)[0], want)
         d_c = ncmul.empty_device_u32((1, N, 28))
         plan.stage_crt(62, ncmul.to_device_u64(want[None]), d_c, 1)
-        assert np.array_equal(ncmul.from_device_u32(d_c)[0], orc.crt_fast(want))
+        assert np.array_equal(ncmul.from_device_u32(d_c)[0], orc.crt_fast(pres))
 
 
-def test_stage_product_residues_and_crt_30(gpu, q868):
+@pytest.mark.parametrize("mfma", [ncmul.MFMA_ALL, 0])
+def test_stage_product_residues_and_crt_30(gpu, q868, mfma):
     N = 1024
-    say("stage product residues 30", N)
+    say("stage product residues 30 mfma", mfma, N)
     qi = O.limbs_to_int(q868)
     a, b = ncmul.gen_uniform(33, N, q868), ncmul.gen_uniform(34, N, q868)
     cint = O.py_negacyclic_exact(O.poly_to_ints(a), O.poly_to_ints(b))
     with ncmul.Plan(N, q868) as plan:
+        plan.set_mfma(mfma)
         pr = [int(p) for p in plan.primes(30)]
-        want = np.array([[c % p for c in cint] for p in pr], dtype=np.uint32)
+        y = crt_weights(pr)
+        want = np.array([[c % p * y[j] % p for c in cint] for j, p in enumerate(pr)], dtype=np.uint32)
         d_a, d_b = ncmul.to_device_u32(a[None]), ncmul.to_device_u32(b[None])
         d_res = ncmul.empty_device_u32((1, plan.k30, N))
         plan.stage_product_residues(30, d_a, d_b, d_res, 1)
@@ -149,7 +178,7 @@ def test_tuning_invariance(gpu, N, width, q868):
         logn = N.bit_length() - 1
         lo, hi = max(0, logn - 14), min(4, logn - 4)
         for lb in range(lo, hi + 1):
-            for mfma in ((1, 0) if width == 30 else (0,)):
+            for mfma in ((ncmul.MFMA_ALL, 0) if width == 30 else (0,)):
                 say("tuning", N, width, lb, "mfma", mfma)
                 plan.set_mfma(mfma)
                 plan.set_tuning(width, lb)
@@ -238,6 +267,51 @@ def test_properties_max_size(gpu, q868):
         sh = O.poly_to_ints(plan.multiply(a, x1))
         ai = O.poly_to_ints(a)
         assert sh[0] == (qi - ai[-1]) % qi and sh[1:] == ai[:-1]
+
+
+def _wide_modulus(L, bits, seed):
+    import random
+    r = random.Random(seed)
+    return (1 << (bits - 1)) | r.getrandbits(bits - 1) | 1
+
+
+@pytest.mark.parametrize("N,B,mod", [
+    (32, 3, "q868"), (64, 2, "q200"), (1024, 1, "q868b"), (16384, 2, "q868"), (64, 3, "w1017"), (32, 1, "w500"),
+])
+def test_crt_mfma_equals_exact(gpu, N, B, mod):
+    """The int8 matrix-core CRT and the VALU CRT against exact Python
+    integers: weighted residues of integers c with |c| <= N (q-1)^2 (the
+    range of negacyclic product coefficients), including both ends, 0 and -1;
+    odd tile counts (B N / 32 odd) leave half a wave idle."""
+    if mod == "w1017":
+        qi, L = _wide_modulus(32, 1017, 9), 32
+    elif mod == "w500":
+        qi, L = _wide_modulus(16, 500, 10), 16
+    else:
+        q = modulus(mod)
+        qi, L = O.limbs_to_int(q), q.shape[0]
+    q = O.int_to_limbs(qi, L)
+    import random
+    r = random.Random(N * 7 + B)
+    bound = N * (qi - 1) ** 2
+    cs = [[r.randrange(-bound, bound + 1) for _ in range(N)] for _ in range(B)]
+    cs[0][:4] = [bound, -bound, 0, -1][: N]
+    with ncmul.Plan(N, q) as plan:
+        assert plan.info.mfma_paths & ncmul.MFMA_CRT
+        pr = [int(p) for p in plan.primes(30)]
+        y = crt_weights(pr)
+        v = np.array([[[c % p * yj % p for c in row] for p, yj in zip(pr, y)] for row in cs], dtype=np.uint32)
+        d_v = ncmul.to_device_u32(v)
+        outs = {}
+        for mode in (ncmul.MFMA_CRT, 0):
+            say("crt", mod, N, B, "mode", mode)
+            plan.set_mfma(mode)
+            d_c = ncmul.empty_device_u32((B, N, L))
+            plan.stage_crt(30, d_v, d_c, B)
+            outs[mode] = ncmul.from_device_u32(d_c)
+    for mode, got in outs.items():
+        for bi in range(B):
+            assert O.poly_to_ints(got[bi]) == [c % qi for c in cs[bi]], (mode, bi)
 
 
 def test_errors(gpu, q868):
