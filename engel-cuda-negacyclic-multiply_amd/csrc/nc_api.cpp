@@ -114,10 +114,13 @@ int check_width(uint32_t width) {
 
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
-    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s,
-                                (c.mfma & kMfmaDecompose) != 0),
+    const bool dm = (c.mfma & kMfmaDecompose) != 0;
+    const bool fuse = nc::decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
+    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, dm,
+                                fuse ? c.log_blocks : 0u),
            "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s, fuse),
+           "ntt launch");
     NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
     return NC_OK;
 }
@@ -481,10 +484,12 @@ int nc_stage_product_residues(nc_plan* p, uint32_t width, const uint32_t* d_a, c
     Choice c = choose(p, batch);
     if (c.width != width) c = heuristic(p, batch, width);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s,
-                                (c.mfma & kMfmaDecompose) != 0),
+    const bool dm = (c.mfma & kMfmaDecompose) != 0;
+    const bool fuse = nc::decompose_fuses_outer(p->dev, (int)width, dm, c.log_blocks);
+    NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s, dm,
+                                fuse ? c.log_blocks : 0u),
            "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s), "ntt launch");
+    NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s, fuse), "ntt launch");
     return NC_OK;
 }
 
@@ -508,6 +513,8 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     hipStream_t s = static_cast<hipStream_t>(stream);
     const Choice c = choose(p, batch);
     const int w = (int)c.width;
+    const bool dm = (c.mfma & kMfmaDecompose) != 0;
+    const bool fuse = nc::decompose_fuses_outer(p->dev, w, dm, c.log_blocks);
     hipEvent_t ev[6];
     for (int i = 0; i < 6; ++i) NC_HIP(hipEventCreate(&ev[i]), "event");
     double acc[5] = {0, 0, 0, 0, 0};
@@ -515,10 +522,11 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
         hipError_t e = hipEventRecord(ev[0], s);
         if (e == hipSuccess)
-            e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s,
-                                     (c.mfma & kMfmaDecompose) != 0);
+            e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, dm,
+                                     fuse ? c.log_blocks : 0u);
         if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-        if (e == hipSuccess) e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
+        if (e == hipSuccess && !fuse)
+            e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[2], s);
         if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[3], s);
@@ -539,6 +547,7 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     const bool outer = p->host.log_n >= 4 && c.log_blocks > 0;
     for (int i = 0; i < 5; ++i) ms[i] = (float)(acc[i] / reps);
     if (!outer) ms[1] = ms[3] = 0.f;
+    if (fuse) ms[1] = 0.f;  // the outer forward stages ran inside the decomposition
     return NC_OK;
 }
 

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "nc_arith.hpp"
 #include "nc_common.hpp"
@@ -164,28 +165,49 @@ typedef int nc_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMfmaCoeffTiles = 4;  // 32-coefficient tiles per wave
 
+template <class A, int R, int NE>
+__device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, int lo, int log_n,
+                                        const typename A::Twid* __restrict__ tw, typename A::T p, typename A::T p2);
+
+// S1 > 0 also runs the first S1 forward NTT stages (the outer stages that
+// otherwise take their own pass over the residue rows): the wave's four tiles
+// are then 4 / 2^S1 column groups of 2^S1 tiles spaced N / 2^S1 apart, so each
+// lane holds, per prime, the 2^S1 values those stages combine.  Output is
+// then lazy in [0, 4p), exactly what ntt_outer_fwd_kernel<W30, S1> leaves.
+template <int S1>
 __global__ __launch_bounds__(256) void decompose30_mfma_kernel(
     const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
     uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int64_t* __restrict__ gtab,
-    const PrimeConst32* __restrict__ pc, uint32_t N, uint32_t L, uint32_t k, uint32_t tiles) {
+    const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
+    uint32_t tiles) {
+    constexpr int G = 1 << S1;                 // spread tiles per column group
+    constexpr int NG = kMfmaCoeffTiles / G;    // column groups per wave
+    static_assert(NG >= 1, "at most 4 spread tiles");
     __shared__ int8_t lds_tab[kMfmaTiles * 32 * kMfmaRowStride];
     struct PrimeLds {
         int64_t g;
         uint32_t p, mont_neg;
     };
     __shared__ PrimeLds lds_prime[kMfmaTiles * 8];
-    constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
+    __shared__ Tw32 lds_tw[kMfmaTiles * 8][G];  // psi^bitrev(x), x = 1 .. G - 1, of each prime
     for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
         const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
         *reinterpret_cast<nc_v4i*>(lds_tab + row * kMfmaRowStride + chunk * 16) =
             *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
     }
-    for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256)
+    for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256) {
         lds_prime[j] = j < k ? PrimeLds{gtab[j], pc[j].p, pc[j].mont_neg} : PrimeLds{0, 1u, 0u};
+        if constexpr (S1 > 0) {
+#pragma unroll
+            for (int x = 1; x < G; ++x) lds_tw[j][x] = j < k ? twf[(size_t)j * N + x] : Tw32{0u, 0u};
+        }
+    }
     __syncthreads();
-    const uint32_t wgs_per_poly = (N + kPerWg - 1) / kPerWg;
+    const uint32_t span = N >> S1;                      // columns of one spread tile group
+    const uint32_t chunks = span / 32;                  // 32-column chunks per polynomial
+    const uint32_t wgs_per_poly = (chunks + 4 * NG - 1) / (4 * NG);
     const uint32_t poly = blockIdx.x / wgs_per_poly;
-    const uint32_t wg_base = (blockIdx.x - poly * wgs_per_poly) * kPerWg;
+    const uint32_t chunk0 = (blockIdx.x - poly * wgs_per_poly) * (4 * NG);
     const bool is_a = poly < na;
     const uint32_t pi = is_a ? poly : poly - na;
     const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
@@ -193,42 +215,63 @@ __global__ __launch_bounds__(256) void decompose30_mfma_kernel(
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 31u, half = lane >> 5;
     const uint32_t nbytes = 4 * L;
-    for (int ct = 0; ct < kMfmaCoeffTiles; ++ct) {
-        const uint32_t c0 = wg_base + (wave * kMfmaCoeffTiles + ct) * 32;
-        if (c0 >= N) break;  // wave-uniform; no barrier follows
-        const uint32_t c = c0 + col;
-        const char* cp = reinterpret_cast<const char*>(src + (size_t)c * L);
-        nc_v4i bf[kMfmaK / 32];
-        uint32_t sa = 0;
+    for (int g = 0; g < NG; ++g) {
+        const uint32_t ch = chunk0 + wave * NG + g;
+        if (ch >= chunks) break;  // wave-uniform; no barrier follows
+        const uint32_t c0 = ch * 32 + col;  // this lane's column; tile e holds coefficient c0 + e span
+        nc_v4i bf[G][kMfmaK / 32];
+        int64_t sa_term[G];
 #pragma unroll
-        for (int ks = 0; ks < kMfmaK / 32; ++ks) {
-            const uint32_t off = 32 * ks + 16 * half;
-            nc_v4i x = {0, 0, 0, 0};
-            if (off < nbytes) x = *reinterpret_cast<const nc_v4i*>(cp + off);
+        for (int e = 0; e < G; ++e) {
+            const char* cp = reinterpret_cast<const char*>(src + (size_t)(c0 + e * span) * L);
+            uint32_t sa = 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sa = __builtin_amdgcn_sad_u8((uint32_t)x[e], 0u, sa);
-            bf[ks] = x ^ (int)0x80808080;
+            for (int ks = 0; ks < kMfmaK / 32; ++ks) {
+                const uint32_t off = 32 * ks + 16 * half;
+                nc_v4i x = {0, 0, 0, 0};
+                if (off < nbytes) x = *reinterpret_cast<const nc_v4i*>(cp + off);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sa = __builtin_amdgcn_sad_u8((uint32_t)x[q], 0u, sa);
+                bf[e][ks] = x ^ (int)0x80808080;
+            }
+            sa += __shfl_xor(sa, 32);
+            sa_term[e] = (int64_t)sa * (128ll * 0x01010101ll);
         }
-        sa += __shfl_xor(sa, 32);
-        const int64_t sa_term = (int64_t)sa * (128ll * 0x01010101ll);
         for (uint32_t t = 0; t < tiles; ++t) {
-            nc_v16i acc = {};
-            const int8_t* arow = lds_tab + (t * 32 + col) * kMfmaRowStride + 16 * half;
+            uint32_t v[4][G];  // [h][e]
 #pragma unroll
-            for (int ks = 0; ks < kMfmaK / 32; ++ks)
-                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*reinterpret_cast<const nc_v4i*>(arow + 32 * ks), bf[ks],
-                                                            acc, 0, 0, 0);
+            for (int e = 0; e < G; ++e) {
+                nc_v16i acc = {};
+                const int8_t* arow = lds_tab + (t * 32 + col) * kMfmaRowStride + 16 * half;
+#pragma unroll
+                for (int ks = 0; ks < kMfmaK / 32; ++ks)
+                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*reinterpret_cast<const nc_v4i*>(arow + 32 * ks),
+                                                                bf[e][ks], acc, 0, 0, 0);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const PrimeLds cj = lds_prime[t * 8 + 2 * h + half];
+                    // |D| < 2^21: the two byte pairs fit int32
+                    const int32_t lo = acc[4 * h] + (int32_t)((uint32_t)acc[4 * h + 1] << 8);
+                    const int32_t hi = acc[4 * h + 2] + (int32_t)((uint32_t)acc[4 * h + 3] << 8);
+                    const uint64_t u =
+                        (uint64_t)((int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sa_term[e] + cj.g);
+                    const uint32_t ul = (uint32_t)u;
+                    const uint32_t r = (uint32_t)(u >> 32) + __umulhi(ul * cj.mont_neg, cj.p) + (ul != 0);
+                    v[h][e] = W30::sub_if_ge(r, cj.p);
+                }
+            }
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const uint32_t j = t * 8 + 2 * h + half;
-                const PrimeLds cj = lds_prime[j];
-                const int64_t v = (int64_t)acc[4 * h] + ((int64_t)acc[4 * h + 1] << 8) +
-                                  ((int64_t)acc[4 * h + 2] << 16) + ((int64_t)acc[4 * h + 3] << 24) + sa_term +
-                                  cj.g;
-                const uint64_t u = (uint64_t)v;
-                const uint32_t lo = (uint32_t)u;
-                const uint32_t r = (uint32_t)(u >> 32) + __umulhi(lo * cj.mont_neg, cj.p) + (lo != 0);
-                if (j < k) dst[(size_t)j * N + c] = W30::sub_if_ge(r, cj.p);
+                if constexpr (S1 > 0) {
+                    const uint32_t p = lds_prime[j].p;
+                    ct_pass<W30, S1, G>(v[h], c0, (int)(31 - __builtin_clz(span)), (int)(31 - __builtin_clz(N)),
+                                        &lds_tw[j][0], p, 2 * p);
+                }
+                if (j < k) {
+#pragma unroll
+                    for (int e = 0; e < G; ++e) dst[(size_t)j * N + c0 + e * span] = v[h][e];
+                }
             }
         }
     }
@@ -439,6 +482,111 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// 30-bit inner kernel: both operands live (16 + 16 words per thread).  The
+// forward transforms of a and b advance stage by stage on the same twiddle
+// loads, their exchanges move (a_x, b_x) pairs as 64-bit LDS words (the
+// padded layouts stay conflict-free at 8 bytes), the pointwise product needs
+// no parking, and the inverse transform runs on the product alone.
+// ---------------------------------------------------------------------------
+template <class A, int R>
+__device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A::T (&vb)[16], uint32_t gx0, int lo,
+                                             int log_n, const typename A::Twid* __restrict__ tw, typename A::T p,
+                                             typename A::T p2) {
+#pragma unroll
+    for (int kk = R - 1; kk >= 0; --kk) {
+        const int beta = lo + kk;
+        const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
+        constexpr int MAXH = 1 << (R - 1);
+        typename A::Twid t[MAXH];
+        const int nh = 1 << (R - 1 - kk);
+#pragma unroll
+        for (int h = 0; h < MAXH; ++h)
+            if (h < nh) t[h] = tw[base + h];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int ep = e & ((1 << R) - 1);
+            if (ep & (1 << kk)) continue;
+            Bfly<A>::ct(va[e], va[e | (1 << kk)], t[ep >> (kk + 1)], p, p2);
+            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], t[ep >> (kk + 1)], p, p2);
+        }
+    }
+}
+
+template <int LOGB, int FROM, int TO>
+__device__ __forceinline__ void exchange_pair(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid) {
+    using F = PassMap<LOGB, FROM>;
+    using G = PassMap<LOGB, TO>;
+    uint2* w = lds + lds_pad(F::base(tid));
+    uint2* r = lds + lds_pad(G::base(tid));
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = make_uint2(va[e], vb[e]);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint2 x = r[lds_pad(G::delta(e))];
+        va[e] = x.x;
+        vb[e] = x.y;
+    }
+}
+
+template <int LOGB, int M>
+__device__ __forceinline__ void forward_pair_from(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid,
+                                                  uint32_t gblk, int log_n, const Tw32* tw, uint32_t p, uint32_t p2) {
+    using P = PassMap<LOGB, M>;
+    if constexpr (M > 0) exchange_pair<LOGB, M - 1, M>(va, vb, lds, tid);
+    ct_pass_pair<W30, P::R>(va, vb, gblk | P::base(tid), P::LO, log_n, tw, p, p2);
+    if constexpr (M + 1 < InnerShape<LOGB>::NP) forward_pair_from<LOGB, M + 1>(va, vb, lds, tid, gblk, log_n, tw, p, p2);
+}
+
+template <int LOGB>
+__global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
+    uint32_t* __restrict__ ra, const uint32_t* __restrict__ rb, const Tw32* __restrict__ twf,
+    const Tw32* __restrict__ twi, const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t k, uint32_t nprod) {
+    using S = InnerShape<LOGB>;
+    using T = uint32_t;
+    constexpr int LAST = S::NP - 1;
+    __shared__ uint2 lds_all[S::BPW * S::PADB];
+    const uint32_t sub = threadIdx.x / S::TPB;
+    const uint32_t tid = threadIdx.x % S::TPB;
+    uint2* lds2 = lds_all + sub * S::PADB;
+    T* lds1 = reinterpret_cast<T*>(lds2);  // the inverse transform reuses the region for one operand
+    const uint32_t N = 1u << log_n;
+    const uint32_t nblk_log = log_n - LOGB;
+    const uint32_t total = k * nprod << nblk_log;
+    uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * S::BPW + sub;
+    if constexpr (S::TPB >= 64) wid = uniform(wid);
+    const bool active = wid < total;
+    if (!active) wid = 0;  // idle sub-block: keeps barriers uniform, touches nothing
+    const uint32_t blk = wid & ((1u << nblk_log) - 1);
+    const uint32_t row = wid >> nblk_log;  // = j * nprod + prod
+    const uint32_t j = row / nprod, prod = row - j * nprod;
+    const size_t off = ((size_t)prod * k + j) * N + ((size_t)blk << LOGB);
+    const uint32_t gblk = blk << LOGB;
+    const PrimeConst32 c = pcs[j];
+    const Tw32* tf = twf + (size_t)j * N;
+    const Tw32* ti = twi + (size_t)j * N;
+    const bool final_out = (nblk_log == 0);
+    using P0 = PassMap<LOGB, 0>;
+    T* ga0 = ra + off + P0::base(tid);
+    const T* gb0 = rb + off + P0::base(tid);
+    T va[16], vb[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        va[e] = active ? ga0[P0::delta(e)] : T(0);
+        vb[e] = active ? gb0[P0::delta(e)] : T(0);
+    }
+    forward_pair_from<LOGB, 0>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
+    __syncthreads();  // the paired region is reused below with the one-operand layout
+    inverse_from<W30, LOGB, LAST>(va, lds1, tid, gblk, (int)log_n, ti, c);
+    if (!active) return;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) ga0[P0::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
+}
+
+// ---------------------------------------------------------------------------
 // outer kernels: the first S1 forward stages / last S1 inverse stages.
 // Thread = one column x = col + (e << b), e in [0, 2^S1), b = logN - S1.
 // ---------------------------------------------------------------------------
@@ -645,15 +793,31 @@ __device__ __forceinline__ void barrett_mod_q(uint32_t (&acc)[LT + 3], const QCo
     }
 }
 
+// acc + K ((-M) mod q) reduced mod q, in place (acc[0 .. LT) on return).
 template <int LT>
 __device__ __forceinline__ void crt_finish(uint32_t (&acc)[LT + 3], uint64_t f_lo, uint64_t f_hi,
-                                           const QConst* __restrict__ qc, uint32_t* __restrict__ o, uint32_t L) {
+                                           const QConst* __restrict__ qc) {
     const uint32_t K = (uint32_t)f_hi + (uint32_t)(f_lo >> 63);
     mac_limbs<LT + 3>(acc, qc->neg_m, LT, K, 0);
     barrett_mod_q<LT>(acc, qc);
+}
+
+// A wave's results leave through LDS: lane i holds the limbs of coefficient
+// base + i (count lanes valid), and dst = out + base L receives count * L
+// consecutive words as coalesced stores instead of one L-word stride per lane.
+template <int NR>
+__device__ __forceinline__ void wave_store_limbs(const uint32_t* r, uint32_t* __restrict__ lds_wave,
+                                                 uint32_t* __restrict__ dst, uint32_t lane, uint32_t L,
+                                                 uint32_t count) {
+    if (lane < count) {
 #pragma unroll
-    for (int l = 0; l < LT; ++l)
-        if ((uint32_t)l < L) o[l] = acc[l];
+        for (int l = 0; l < NR; ++l)
+            if ((uint32_t)l < L) lds_wave[lane * L + l] = r[l];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t words = count * L;
+    for (uint32_t w = lane; w < words; w += 64u) dst[w] = lds_wave[w];
 }
 
 template <int LT>
@@ -662,10 +826,15 @@ __global__ __launch_bounds__(256) void crt62_kernel(const uint64_t* __restrict__
                                                     const QConst* __restrict__ qc, const PrimeConst* __restrict__ pcs,
                                                     uint32_t log_n, uint32_t L, uint32_t k, uint32_t nprod) {
     constexpr int NA = LT + 3;
+    __shared__ uint32_t lds_out[256 * LT];
     const uint32_t N = 1u << log_n;
     const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (nprod << log_n)) return;
-    const uint32_t prod = idx >> log_n, i = idx & (N - 1);
+    const uint32_t total = nprod << log_n;
+    const uint32_t lane = threadIdx.x & 63u, wbase = idx - lane;
+    if (wbase >= total) return;  // wave-uniform
+    const uint32_t count = total - wbase < 64u ? total - wbase : 64u;
+    const uint32_t ic = idx < total ? idx : wbase;  // idle lanes recompute a live coefficient, store nothing
+    const uint32_t prod = ic >> log_n, i = ic & (N - 1);
     const uint64_t* r = res + (size_t)prod * k * N + i;
     uint32_t acc[NA];
 #pragma unroll
@@ -689,7 +858,8 @@ __global__ __launch_bounds__(256) void crt62_kernel(const uint64_t* __restrict__
             mac_limbs<NA>(acc, m, LT, (uint32_t)(v >> 32), 1);
         }
     }
-    crt_finish<LT>(acc, f_lo, f_hi, qc, out + ((size_t)prod * N + i) * L, L);
+    crt_finish<LT>(acc, f_lo, f_hi, qc);
+    wave_store_limbs<LT>(acc, lds_out + (threadIdx.x - lane) * LT, out + (size_t)wbase * L, lane, L, count);
 }
 
 // 30-bit primes: v_j < 2^30, so v_j m_l < 2^62 and three products fit a
@@ -701,10 +871,15 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
                                                     const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t L,
                                                     uint32_t k, uint32_t nprod) {
     constexpr int NA = LT + 3;
+    __shared__ uint32_t lds_out[256 * LT];
     const uint32_t N = 1u << log_n;
     const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (nprod << log_n)) return;
-    const uint32_t prod = idx >> log_n, i = idx & (N - 1);
+    const uint32_t total = nprod << log_n;
+    const uint32_t lane = threadIdx.x & 63u, wbase = idx - lane;
+    if (wbase >= total) return;  // wave-uniform
+    const uint32_t count = total - wbase < 64u ? total - wbase : 64u;
+    const uint32_t ic = idx < total ? idx : wbase;  // idle lanes recompute a live coefficient, store nothing
+    const uint32_t prod = ic >> log_n, i = ic & (N - 1);
     const uint32_t* r = res + (size_t)prod * k * N + i;
     uint64_t col[LT + 1];
 #pragma unroll
@@ -747,7 +922,8 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
         acc[l] = (uint32_t)t;
         carry = t >> 32;
     }
-    crt_finish<LT>(acc, f_lo, f_hi, qc, out + ((size_t)prod * N + i) * L, L);
+    crt_finish<LT>(acc, f_lo, f_hi, qc);
+    wave_store_limbs<LT>(acc, lds_out + (threadIdx.x - lane) * LT, out + (size_t)wbase * L, lane, L, count);
 }
 
 // x -> x 2^-96 mod q for x < q 2^96 (NA limbs), three word-level Montgomery
@@ -923,7 +1099,7 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             if ((uint32_t)l < L) st[lane * L + l] = r[l];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t words = (total - g0 < 64u ? total - g0 : 64u) * L;  // a multiple of 4 (32 | count)
+        const uint32_t words = (total - g0 < 64u ? total - g0 : 64u) * L;  // a multiple of 4 (32 divides the count)
         uint32_t* dst = out + (size_t)g0 * L;
         for (uint32_t w = lane * 4u; w < words; w += 256u)
             *reinterpret_cast<nc_v4i*>(dst + w) = *reinterpret_cast<const nc_v4i*>(st + w);
@@ -1000,8 +1176,12 @@ static void launch_inner(const DevPlan& d, void* ra, const void* rb, uint32_t np
     const auto& set = SetOf<A>::get(d);
     const uint32_t items = (set.k * nprod) << (d.log_n - LOGB);
     const uint32_t grid = (items + S::BPW - 1) / S::BPW;
-    ntt_inner_kernel<A, LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                      set.tw_inv, set.pc, d.log_n, set.k, nprod);
+    if constexpr (std::is_same<A, W30>::value && LOGB >= 5)
+        ntt_inner30_kernel<LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
+                                                         set.tw_inv, set.pc, d.log_n, set.k, nprod);
+    else
+        ntt_inner_kernel<A, LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
+                                                          set.tw_inv, set.pc, d.log_n, set.k, nprod);
 }
 
 template <class A>
@@ -1081,8 +1261,15 @@ static hipError_t outer_inv_dispatch(const DevPlan& d, void* ra, uint32_t nprod,
     return hipGetLastError();
 }
 
+bool decompose_fuses_outer(const DevPlan& d, int width, bool mfma, uint32_t log_blocks) {
+    return width == 30 && mfma && d.w30.mfma_tab && log_blocks >= 1 && log_blocks <= 2 && d.log_n >= 4 &&
+           (d.N >> log_blocks) >= 32;
+}
+
 hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void* res_a, uint32_t na,
-                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma) {
+                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma,
+                            uint32_t fuse_s1) {
+    if (fuse_s1 && !decompose_fuses_outer(d, width, mfma, fuse_s1)) return hipErrorInvalidValue;
     const dim3 grid(((d.N + 63) / 64) * (na + nb));
     if (width == 62) {
         uint64_t* ra = static_cast<uint64_t*>(res_a);
@@ -1095,9 +1282,24 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
     } else if (mfma && d.w30.mfma_tab && d.N >= 32) {
         constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
         const dim3 g(((d.N + kPerWg - 1) / kPerWg) * (na + nb));
-        decompose30_mfma_kernel<<<g, 256, 0, s>>>(a, static_cast<uint32_t*>(res_a), na, b,
-                                                  static_cast<uint32_t*>(res_b), d.w30.mfma_tab, d.w30.mfma_g,
-                                                  d.w30.pc, d.N, d.L, d.w30.k, d.w30.mfma_tiles);
+        uint32_t* ra = static_cast<uint32_t*>(res_a);
+        uint32_t* rb = static_cast<uint32_t*>(res_b);
+        const DevSet30& w = d.w30;
+        switch (fuse_s1) {
+            case 0:
+                decompose30_mfma_kernel<0><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                                                             d.N, d.L, w.k, w.mfma_tiles);
+                break;
+            case 1:
+                decompose30_mfma_kernel<1><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                                                             d.N, d.L, w.k, w.mfma_tiles);
+                break;
+            case 2:
+                decompose30_mfma_kernel<2><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                                                             d.N, d.L, w.k, w.mfma_tiles);
+                break;
+            default: return hipErrorInvalidValue;
+        }
     } else {
         uint32_t* ra = static_cast<uint32_t*>(res_a);
         uint32_t* rb = static_cast<uint32_t*>(res_b);
@@ -1129,10 +1331,10 @@ hipError_t launch_ntt_outer_inv(const DevPlan& d, int width, void* ra, uint32_t 
 }
 
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
-                              hipStream_t s) {
+                              hipStream_t s, bool outer_fwd_done) {
     if (d.log_n >= 4 && (log_blocks < min_log_blocks(d.log_n) || log_blocks > max_log_blocks(d.log_n)))
         return hipErrorInvalidValue;
-    hipError_t e = launch_ntt_outer_fwd(d, width, ra, rb, nprod, log_blocks, s);
+    hipError_t e = outer_fwd_done ? hipSuccess : launch_ntt_outer_fwd(d, width, ra, rb, nprod, log_blocks, s);
     if (e != hipSuccess) return e;
     e = launch_ntt_inner(d, width, ra, rb, nprod, log_blocks, s);
     if (e != hipSuccess) return e;

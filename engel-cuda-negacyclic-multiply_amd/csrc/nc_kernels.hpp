@@ -56,8 +56,12 @@ struct DevPlan {
 // limbs -> canonical residues; polys a (na of them) then b (nb) in one launch.
 // mfma: use the int8 matrix-core contraction where available (30-bit primes,
 // L % 4 == 0, N >= 32); otherwise, and when false, the VALU kernels.
+// fuse_s1 > 0 (only where decompose_fuses_outer) also runs the first fuse_s1
+// forward NTT stages, leaving what launch_ntt_outer_fwd would.
+bool decompose_fuses_outer(const DevPlan& d, int width, bool mfma, uint32_t log_blocks);
 hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void* res_a, uint32_t na,
-                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma = true);
+                            const uint32_t* b, void* res_b, uint32_t nb, hipStream_t s, bool mfma = true,
+                            uint32_t fuse_s1 = 0);
 
 // Residue rows of a and b -> CRT-weighted residues of the exact negacyclic
 // product, v_j = (c mod p_j) (M / p_j)^-1 mod p_j (the inverse transform's
@@ -65,7 +69,7 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
 // [0, p)); rb is clobbered.  log_blocks is the outer
 // split (0 = the whole row in one workgroup).
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
-                              hipStream_t s);
+                              hipStream_t s, bool outer_fwd_done = false);
 
 // The three parts of launch_ntt_product, for per-stage timing.
 hipError_t launch_ntt_outer_fwd(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
