@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/negacyclic.h"
+#include "nc_internal.hpp"
 #include "nc_kernels.hpp"
 #include "nc_plan.hpp"
 
@@ -114,15 +115,7 @@ int check_width(uint32_t width) {
 
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
-    const bool dm = (c.mfma & kMfmaDecompose) != 0;
-    const bool fuse = nc::decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
-    NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, dm,
-                                fuse ? c.log_blocks : 0u),
-           "decompose launch");
-    NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, p->d_res_a, p->d_res_b, batch, c.log_blocks, s, fuse),
-           "ntt launch");
-    NC_HIP(nc::launch_crt(p->dev, (int)c.width, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
-    return NC_OK;
+    return nc::run_pipeline_ws(p, p->d_res_a, p->d_res_b, d_a, d_b, d_c, batch, s, c);
 }
 
 // Upload one host table vector at `off` of the table allocation.
@@ -132,6 +125,32 @@ hipError_t upload(char* base, size_t off, const V& v) {
 }
 
 }  // namespace
+
+namespace nc {
+
+int set_error(int status, const std::string& msg) { return fail(status, msg); }
+
+Choice choose_for(const nc_plan* p, uint32_t batch) { return choose(p, batch); }
+
+size_t workspace_bytes_per_product(const nc_plan* p) { return p->res_bytes; }
+
+int run_pipeline_ws(nc_plan* p, void* ws_a, void* ws_b, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                    uint32_t batch, hipStream_t s, const Choice& c) {
+    const bool dm = (c.mfma & kMfmaDecompose) != 0;
+    const bool fuse = decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
+    NC_HIP(launch_decompose(p->dev, (int)c.width, d_a, ws_a, batch, d_b, ws_b, batch, s, dm, fuse ? c.log_blocks : 0u),
+           "decompose launch");
+    NC_HIP(launch_ntt_product(p->dev, (int)c.width, ws_a, ws_b, batch, c.log_blocks, s, fuse), "ntt launch");
+    NC_HIP(launch_crt(p->dev, (int)c.width, ws_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
+    return NC_OK;
+}
+
+int ensure_workspace(nc_plan* p, uint32_t batch) {
+    if (int rc = check_batch(p, batch)) return rc;
+    return grow_workspace(p, batch);
+}
+
+}  // namespace nc
 
 extern "C" {
 

@@ -1,0 +1,392 @@
+// Serving loop (include/negacyclic_batch.h): continuous batching of single
+// product requests of mixed ring degree over resident per-degree plans.
+//
+// One scheduler thread owns the device work.  Per degree it keeps a queue of
+// waiting requests and `lanes_per_degree` lanes; a lane is one in-flight
+// batch with its own stream, event, residue workspace and device staging for
+// max_batch products.  Each turn the scheduler (1) moves newly submitted
+// requests to their degree's queue, (2) retires lanes whose event has fired
+// (requests complete, their device bytes are released), and (3) fills every
+// idle lane with waiting requests of its degree, as many as max_batch and the
+// admission budget allow: H2D copies of the operands, the batched pipeline,
+// D2H copies of the results, then the lane's event, all on the lane's
+// stream, so copies of one batch overlap kernels of others.  Requests that
+// arrive while lanes are busy are admitted as soon as a lane and budget free
+// up (the next turn), not after the queue drains.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/negacyclic_batch.h"
+#include "nc_internal.hpp"
+
+namespace {
+
+using nc::set_error;
+using Clock = std::chrono::steady_clock;
+
+double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+
+struct Req {
+    uint64_t id = 0;
+    uint32_t N = 0;
+    const uint32_t* a = nullptr;
+    const uint32_t* b = nullptr;
+    uint32_t* c = nullptr;
+    uint64_t bytes = 0;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int status = NC_OK;
+};
+
+struct Lane {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    void* ws_a = nullptr;
+    void* ws_b = nullptr;
+    uint32_t* io = nullptr;  // a | b | c, cap products each
+    std::vector<Req*> batch;
+    uint64_t bytes = 0;
+    bool busy = false;
+};
+
+struct Degree {
+    uint32_t N = 0;
+    nc_plan* plan = nullptr;
+    uint64_t footprint = 0;
+    uint32_t cap = 0;
+    std::deque<Req*> queue;
+    std::vector<Lane> lanes;
+};
+
+}  // namespace
+
+struct nc_server {
+    nc_server_config cfg{};
+    std::vector<uint32_t> q;
+    int device = 0;
+    std::map<uint32_t, Degree> deg;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req*> inbox;                  // under mu
+    std::unordered_map<uint64_t, Req*> reqs;  // under mu
+    uint64_t next_id = 1;                     // under mu
+    bool stop = false;                        // under mu
+    nc_server_stats st{};                     // under mu
+    uint64_t inflight = 0;                    // scheduler thread only
+    std::thread th;
+};
+
+namespace {
+
+int hip_err(hipError_t e, const char* what) { return set_error(NC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)); }
+
+void finish(nc_server* sv, Lane& ln, int status) {
+    const double t = now_s();
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        for (Req* r : ln.batch) {
+            r->t_done = t;
+            r->status = status;
+            r->done = true;
+        }
+        sv->st.completed += ln.batch.size();
+    }
+    sv->cv_done.notify_all();
+    sv->inflight -= ln.bytes;
+    ln.bytes = 0;
+    ln.batch.clear();
+    ln.busy = false;
+}
+
+// Enqueues one batch on an idle lane; returns false if nothing was admitted.
+bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
+    const uint64_t budget = sv->cfg.hbm_budget;
+    while (!d.queue.empty() && ln.batch.size() < d.cap && sv->inflight + d.footprint <= budget) {
+        ln.batch.push_back(d.queue.front());
+        d.queue.pop_front();
+        sv->inflight += d.footprint;
+        ln.bytes += d.footprint;
+    }
+    if (ln.batch.empty()) return false;
+    const uint32_t n = (uint32_t)ln.batch.size();
+    const size_t words = (size_t)d.N * sv->cfg.L;
+    uint32_t* da = ln.io;
+    uint32_t* db = ln.io + (size_t)d.cap * words;
+    uint32_t* dc = ln.io + 2 * (size_t)d.cap * words;
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
+        e = hipMemcpyAsync(da + i * words, ln.batch[i]->a, words * 4, hipMemcpyHostToDevice, ln.s);
+        if (e == hipSuccess) e = hipMemcpyAsync(db + i * words, ln.batch[i]->b, words * 4, hipMemcpyHostToDevice, ln.s);
+    }
+    int rc = e == hipSuccess ? NC_OK : hip_err(e, "request copy in");
+    if (rc == NC_OK) rc = nc::run_pipeline_ws(d.plan, ln.ws_a, ln.ws_b, da, db, dc, n, ln.s, nc::choose_for(d.plan, n));
+    for (uint32_t i = 0; i < n && rc == NC_OK; ++i) {
+        e = hipMemcpyAsync(ln.batch[i]->c, dc + i * words, words * 4, hipMemcpyDeviceToHost, ln.s);
+        if (e != hipSuccess) rc = hip_err(e, "request copy out");
+    }
+    if (rc == NC_OK) {
+        e = hipEventRecord(ln.ev, ln.s);
+        if (e != hipSuccess) rc = hip_err(e, "lane event");
+    }
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->st.launches += 1;
+        sv->st.max_batch_seen = std::max<uint64_t>(sv->st.max_batch_seen, n);
+        sv->st.peak_bytes = std::max<uint64_t>(sv->st.peak_bytes, sv->inflight);
+    }
+    if (rc != NC_OK) {
+        (void)hipStreamSynchronize(ln.s);
+        finish(sv, ln, rc);
+        return true;
+    }
+    ln.busy = true;
+    return true;
+}
+
+void scheduler(nc_server* sv) {
+    (void)hipSetDevice(sv->device);
+    for (;;) {
+        bool any_busy = false, waiting = false;
+        {
+            std::lock_guard<std::mutex> g(sv->mu);
+            while (!sv->inbox.empty()) {
+                Req* r = sv->inbox.front();
+                sv->inbox.pop_front();
+                sv->deg[r->N].queue.push_back(r);
+            }
+        }
+        bool progress = false;
+        for (auto& kv : sv->deg)
+            for (Lane& ln : kv.second.lanes) {
+                if (!ln.busy) continue;
+                const hipError_t q = hipEventQuery(ln.ev);
+                if (q == hipSuccess) {
+                    finish(sv, ln, NC_OK);
+                    progress = true;
+                } else if (q != hipErrorNotReady) {
+                    finish(sv, ln, hip_err(q, "lane"));
+                    progress = true;
+                }
+            }
+        for (auto& kv : sv->deg) {
+            Degree& d = kv.second;
+            for (Lane& ln : d.lanes) {
+                if (d.queue.empty()) break;
+                if (ln.busy) continue;
+                if (dispatch(sv, d, ln)) {
+                    progress = true;
+                } else {
+                    std::lock_guard<std::mutex> g(sv->mu);
+                    sv->st.admission_waits += 1;
+                    break;
+                }
+            }
+            for (Lane& ln : d.lanes) any_busy = any_busy || ln.busy;
+            waiting = waiting || !d.queue.empty();
+        }
+        std::unique_lock<std::mutex> lk(sv->mu);
+        if (sv->stop && !any_busy && !waiting && sv->inbox.empty()) break;
+        if (!progress && sv->inbox.empty()) {
+            // poll in-flight lanes often; sleep longer when idle (a submit wakes us)
+            sv->cv_work.wait_for(lk, any_busy ? std::chrono::microseconds(20) : std::chrono::milliseconds(2));
+        }
+    }
+}
+
+void free_degree(Degree& d) {
+    for (Lane& ln : d.lanes) {
+        if (ln.s) (void)hipStreamSynchronize(ln.s);
+        if (ln.ev) (void)hipEventDestroy(ln.ev);
+        if (ln.s) (void)hipStreamDestroy(ln.s);
+        if (ln.ws_a) (void)hipFree(ln.ws_a);
+        if (ln.ws_b) (void)hipFree(ln.ws_b);
+        if (ln.io) (void)hipFree(ln.io);
+    }
+    if (d.plan) nc_plan_destroy(d.plan);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* nc_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+
+void nc_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
+void nc_server_destroy(nc_server* sv) {
+    if (!sv) return;
+    if (sv->th.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(sv->mu);
+            sv->stop = true;
+        }
+        sv->cv_work.notify_all();
+        sv->th.join();
+    }
+    for (auto& kv : sv->deg) free_degree(kv.second);
+    for (auto& kv : sv->reqs) delete kv.second;
+    delete sv;
+}
+
+int nc_server_create(const nc_server_config* cfg, nc_server** out) {
+    if (!out) return set_error(NC_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (!cfg || !cfg->q || !cfg->degrees || cfg->n_degrees == 0) return set_error(NC_ERR_INVALID, "bad config");
+    if (cfg->L == 0 || cfg->L > 32) return set_error(NC_ERR_INVALID, "L must be in [1, 32]");
+    if (cfg->max_batch == 0 || cfg->lanes_per_degree == 0) return set_error(NC_ERR_INVALID, "max_batch and lanes must be >= 1");
+    nc_server* sv = new (std::nothrow) nc_server();
+    if (!sv) return set_error(NC_ERR_NOMEM, "server");
+    sv->cfg = *cfg;
+    sv->q.assign(cfg->q, cfg->q + cfg->L);
+    sv->cfg.q = sv->q.data();
+    sv->cfg.degrees = nullptr;
+    auto fail_out = [&](int rc) {
+        nc_server_destroy(sv);
+        return rc;
+    };
+    hipError_t e = hipGetDevice(&sv->device);
+    if (e != hipSuccess) return fail_out(hip_err(e, "hipGetDevice"));
+    for (uint32_t i = 0; i < cfg->n_degrees; ++i) {
+        const uint32_t N = cfg->degrees[i];
+        if (sv->deg.count(N)) continue;
+        Degree& d = sv->deg[N];
+        d.N = N;
+        if (int rc = nc_plan_create(N, cfg->L, sv->q.data(), &d.plan)) return fail_out(rc);
+        const size_t words = (size_t)N * cfg->L;
+        const size_t ws = nc::workspace_bytes_per_product(d.plan);
+        d.footprint = 3 * words * 4 + 2 * ws;
+        if (d.footprint > cfg->hbm_budget)
+            return fail_out(set_error(NC_ERR_CAPACITY, "hbm_budget below one request of degree " + std::to_string(N)));
+        if ((uint64_t)cfg->max_batch * N > (1ull << 31)) return fail_out(set_error(NC_ERR_CAPACITY, "max_batch * N above 2^31"));
+        d.cap = cfg->max_batch;
+        if (cfg->autotune) {
+            for (uint32_t bsz : {1u, d.cap})
+                if (int rc = nc_plan_autotune(d.plan, bsz, nullptr, nullptr, nullptr)) return fail_out(rc);
+        }
+        d.lanes.resize(cfg->lanes_per_degree);
+        for (Lane& ln : d.lanes) {
+            e = hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipMalloc(&ln.ws_a, (size_t)d.cap * ws);
+            if (e == hipSuccess) e = hipMalloc(&ln.ws_b, (size_t)d.cap * ws);
+            if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ln.io), 3 * (size_t)d.cap * words * 4);
+            if (e != hipSuccess) return fail_out(hip_err(e, "lane allocation"));
+            sv->st.device_bytes += 2 * (size_t)d.cap * ws + 3 * (size_t)d.cap * words * 4;
+        }
+    }
+    sv->th = std::thread(scheduler, sv);
+    *out = sv;
+    return NC_OK;
+}
+
+int nc_server_footprint(const nc_server* sv, uint32_t N, uint64_t* bytes) {
+    if (!sv || !bytes) return set_error(NC_ERR_INVALID, "null argument");
+    auto it = sv->deg.find(N);
+    if (it == sv->deg.end()) return set_error(NC_ERR_INVALID, "degree not served");
+    *bytes = it->second.footprint;
+    return NC_OK;
+}
+
+int nc_server_submit(nc_server* sv, uint32_t N, const uint32_t* a, const uint32_t* b, uint32_t* c, uint64_t* ticket) {
+    if (!sv || !a || !b || !c || !ticket) return set_error(NC_ERR_INVALID, "null argument");
+    auto it = sv->deg.find(N);  // the degree map is fixed after creation
+    if (it == sv->deg.end()) return set_error(NC_ERR_INVALID, "degree " + std::to_string(N) + " not served");
+    Req* r = new (std::nothrow) Req();
+    if (!r) return set_error(NC_ERR_NOMEM, "request");
+    r->N = N;
+    r->a = a;
+    r->b = b;
+    r->c = c;
+    r->bytes = it->second.footprint;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        if (sv->stop) {
+            delete r;
+            return set_error(NC_ERR_STATE, "server is stopping");
+        }
+        r->id = sv->next_id++;
+        sv->reqs[r->id] = r;
+        sv->inbox.push_back(r);
+        sv->st.submitted += 1;
+    }
+    sv->cv_work.notify_one();
+    *ticket = r->id;
+    return NC_OK;
+}
+
+int nc_server_wait(nc_server* sv, uint64_t ticket, int timeout_ms, double* latency_s) {
+    if (!sv) return set_error(NC_ERR_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return set_error(NC_ERR_STATE, "unknown ticket");
+    Req* r = it->second;
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return set_error(NC_ERR_STATE, "timed out");
+    if (latency_s) *latency_s = r->t_done - r->t_submit;
+    const int status = r->status;
+    sv->reqs.erase(it);
+    delete r;
+    return status;
+}
+
+int nc_server_get_stats(const nc_server* sv, nc_server_stats* out) {
+    if (!sv || !out) return set_error(NC_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(const_cast<nc_server*>(sv)->mu);
+    *out = sv->st;
+    return NC_OK;
+}
+
+int nc_server_replay(nc_server* sv, uint32_t n, const uint32_t* degrees, const double* arrival_s,
+                     const uint32_t* const* a, const uint32_t* const* b, uint32_t* const* c, double* latency_s,
+                     double* makespan_s) {
+    if (!sv || !degrees || !arrival_s || !a || !b || !c || !latency_s) return set_error(NC_ERR_INVALID, "null argument");
+    std::vector<uint64_t> tickets(n);
+    std::vector<double> sched(n);
+    const double t0 = now_s();
+    int rc = NC_OK;
+    uint32_t sent = 0;
+    for (; sent < n; ++sent) {
+        if (sent && arrival_s[sent] < arrival_s[sent - 1]) {
+            rc = set_error(NC_ERR_INVALID, "arrival times must ascend");
+            break;
+        }
+        sched[sent] = t0 + arrival_s[sent];
+        while (now_s() < sched[sent]) std::this_thread::yield();
+        rc = nc_server_submit(sv, degrees[sent], a[sent], b[sent], c[sent], &tickets[sent]);
+        if (rc) break;
+    }
+    double last = t0;
+    for (uint32_t i = 0; i < sent; ++i) {  // every submitted request is waited for, even after an error
+        std::unique_lock<std::mutex> lk(sv->mu);
+        Req* r = sv->reqs[tickets[i]];
+        sv->cv_done.wait(lk, [&] { return r->done; });
+        latency_s[i] = r->t_done - sched[i];
+        last = std::max(last, r->t_done);
+        if (rc == NC_OK && r->status != NC_OK) rc = r->status;
+        sv->reqs.erase(tickets[i]);
+        delete r;
+    }
+    if (makespan_s) *makespan_s = last - t0;
+    return rc;
+}
+
+}  // extern "C"

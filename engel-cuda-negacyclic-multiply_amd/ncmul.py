@@ -46,6 +46,24 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+class Group(ctypes.Structure):
+    """nc_group: `count` products of degree N, contiguous device operands."""
+    _fields_ = [("N", ctypes.c_uint32), ("count", ctypes.c_uint32), ("d_a", _vp), ("d_b", _vp), ("d_c", _vp)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("L", ctypes.c_uint32), ("q", _u32p), ("degrees", _u32p), ("n_degrees", ctypes.c_uint32),
+        ("max_batch", ctypes.c_uint32), ("lanes_per_degree", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+        ("autotune", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "launches", "max_batch_seen", "peak_bytes",
+                                                "admission_waits", "device_bytes")]
+
+
 _lib = None
 
 
@@ -80,6 +98,31 @@ def lib():
         L.nc_profile_stages.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
                                         ctypes.POINTER(ctypes.c_float)]
         L.nc_measure_mul32_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        # multi-degree batches and serving (include/negacyclic_batch.h)
+        L.nc_batch_create.argtypes = [ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)]
+        L.nc_batch_destroy.argtypes = [_vp]
+        L.nc_batch_destroy.restype = None
+        L.nc_batch_plan.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.nc_batch_autotune.argtypes = [_vp, ctypes.POINTER(Group), ctypes.c_uint32, _vp]
+        L.nc_batch_run.argtypes = [_vp, ctypes.POINTER(Group), ctypes.c_uint32, _vp]
+        L.nc_batch_capture.argtypes = [_vp, ctypes.POINTER(Group), ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.nc_graph_launch.argtypes = [_vp, _vp]
+        L.nc_graph_destroy.argtypes = [_vp]
+        L.nc_graph_destroy.restype = None
+        L.nc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]
+        L.nc_server_destroy.argtypes = [_vp]
+        L.nc_server_destroy.restype = None
+        L.nc_server_footprint.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
+        L.nc_server_submit.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.nc_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.nc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.nc_server_replay.argtypes = [_vp, ctypes.c_uint32, _u32p, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.nc_host_alloc.argtypes = [ctypes.c_size_t]
+        L.nc_host_alloc.restype = _vp
+        L.nc_host_free.argtypes = [_vp]
+        L.nc_host_free.restype = None
         _lib = L
     return _lib
 
@@ -261,3 +304,171 @@ def empty_device_u64(shape, device="cuda"):
     import torch
 
     return torch.empty(shape, dtype=torch.int64, device=device)
+
+
+# ---------------------------------------------------------------------------
+# pinned host arrays, multi-degree batches, serving
+# ---------------------------------------------------------------------------
+def host_array(shape, dtype=np.uint32) -> np.ndarray:
+    """numpy array in pinned host memory (nc_host_alloc), freed with the array."""
+    import weakref
+
+    count = int(np.prod(shape))
+    nbytes = max(1, count * np.dtype(dtype).itemsize)
+    ptr = lib().nc_host_alloc(nbytes)
+    if not ptr:
+        raise MemoryError(f"nc_host_alloc({nbytes}) failed")
+    buf = (ctypes.c_char * nbytes).from_address(ptr)
+    arr = np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+    weakref.finalize(buf, lib().nc_host_free, ctypes.c_void_p(ptr))
+    return arr
+
+
+def _groups(groups):
+    """[(N, count, d_a, d_b, d_c)] with torch device tensors -> nc_group array."""
+    arr = (Group * len(groups))()
+    for i, (N, count, a, b, c) in enumerate(groups):
+        arr[i] = Group(N, count, a.data_ptr(), b.data_ptr(), c.data_ptr())
+    return arr
+
+
+class Graph:
+    """An executable HIP graph of one multi-degree batch (nc_batch_capture)."""
+
+    def __init__(self, handle, keep):
+        self._h = handle
+        self._keep = keep  # the bound device buffers stay alive with the graph
+
+    def launch(self, stream=None):
+        _check(lib().nc_graph_launch(self._h, _stream_handle(stream)), "nc_graph_launch")
+
+    def close(self):
+        if self._h:
+            lib().nc_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Resident per-degree plans for one modulus; runs or captures groups of
+    products of mixed degree.  Groups are (N, count, d_a, d_b, d_c) with
+    torch device tensors of shape (count, N, L)."""
+
+    def __init__(self, q: np.ndarray):
+        self.q = np.ascontiguousarray(q, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        _check(lib().nc_batch_create(self.q.shape[0], _p32(self.q), ctypes.byref(h)), "nc_batch_create")
+        self._h = h
+        self._graphs = []
+
+    def autotune(self, groups, stream=None):
+        g = _groups(groups)
+        _check(lib().nc_batch_autotune(self._h, g, len(groups), _stream_handle(stream)), "nc_batch_autotune")
+
+    def run(self, groups, stream=None):
+        g = _groups(groups)
+        _check(lib().nc_batch_run(self._h, g, len(groups), _stream_handle(stream)), "nc_batch_run")
+
+    def capture(self, groups) -> Graph:
+        g = _groups(groups)
+        h = ctypes.c_void_p()
+        _check(lib().nc_batch_capture(self._h, g, len(groups), ctypes.byref(h)), "nc_batch_capture")
+        graph = Graph(h, list(groups))
+        self._graphs.append(graph)
+        return graph
+
+    def close(self):
+        for g in self._graphs:
+            g.close()
+        self._graphs = []
+        if self._h:
+            lib().nc_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching server over resident per-degree plans (nc_server_*).
+    Request buffers should come from host_array (pinned)."""
+
+    def __init__(self, q: np.ndarray, degrees, max_batch: int = 32, lanes: int = 2, hbm_budget: int = 8 << 30,
+                 autotune: bool = True):
+        self.q = np.ascontiguousarray(q, dtype=np.uint32)
+        self.L = self.q.shape[0]
+        self.degrees = np.ascontiguousarray(sorted(set(int(n) for n in degrees)), dtype=np.uint32)
+        cfg = ServerConfig(self.L, _p32(self.q), _p32(self.degrees), len(self.degrees), max_batch, lanes,
+                           hbm_budget, 1 if autotune else 0)
+        h = ctypes.c_void_p()
+        _check(lib().nc_server_create(ctypes.byref(cfg), ctypes.byref(h)), "nc_server_create")
+        self._h = h
+        self._live = {}  # ticket -> buffers kept alive until waited for
+
+    def footprint(self, N: int) -> int:
+        out = ctypes.c_uint64()
+        _check(lib().nc_server_footprint(self._h, N, ctypes.byref(out)), "nc_server_footprint")
+        return int(out.value)
+
+    def submit(self, a: np.ndarray, b: np.ndarray, c: np.ndarray) -> int:
+        N = a.shape[0]
+        for x in (a, b, c):
+            assert x.dtype == np.uint32 and x.flags.c_contiguous and x.shape == (N, self.L)
+        t = ctypes.c_uint64()
+        _check(lib().nc_server_submit(self._h, N, a.ctypes.data, b.ctypes.data, c.ctypes.data, ctypes.byref(t)),
+               "nc_server_submit")
+        self._live[t.value] = (a, b, c)
+        return int(t.value)
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> float:
+        lat = ctypes.c_double()
+        rc = lib().nc_server_wait(self._h, ticket, timeout_ms, ctypes.byref(lat))
+        self._live.pop(ticket, None)
+        _check(rc, "nc_server_wait")
+        return lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(lib().nc_server_get_stats(self._h, ctypes.byref(st)), "nc_server_get_stats")
+        return {n: int(getattr(st, n)) for n, _ in ServerStats._fields_}
+
+    def replay(self, degrees, arrivals, a_list, b_list, c_list):
+        """Open-loop replay; returns (latencies seconds, makespan seconds)."""
+        n = len(degrees)
+        deg = np.ascontiguousarray(degrees, dtype=np.uint32)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        pa = (_vp * n)(*[x.ctypes.data for x in a_list])
+        pb = (_vp * n)(*[x.ctypes.data for x in b_list])
+        pc = (_vp * n)(*[x.ctypes.data for x in c_list])
+        lat = np.zeros(n, dtype=np.float64)
+        mk = ctypes.c_double()
+        _check(lib().nc_server_replay(self._h, n, _p32(deg), arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      pa, pb, pc, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      ctypes.byref(mk)), "nc_server_replay")
+        return lat, mk.value
+
+    def close(self):
+        if self._h:
+            lib().nc_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,120 @@
+/* Multi-degree batches and serving on top of the negacyclic product library
+ * (negacyclic.h).  Neither has a counterpart in the reference, whose only
+ * entry points run one product per call (fherma.h:45-47, solve.cu:1489-1528);
+ * they are the batched and served forms of that same product, bit-identical
+ * to it per product.
+ *
+ * Thread safety: an nc_batch is used from one host thread at a time; an
+ * nc_server may be called from any number of threads. */
+#ifndef NEGACYCLIC_BATCH_H
+#define NEGACYCLIC_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "negacyclic.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- batches of mixed ring degree --------------------------------------
+ * A group is `count` products of one degree N whose operands lie contiguous
+ * in device memory (count x N x L u32 each, product-major). */
+typedef struct nc_group {
+    uint32_t N;
+    uint32_t count;
+    const uint32_t* d_a;
+    const uint32_t* d_b;
+    uint32_t* d_c;
+} nc_group;
+
+typedef struct nc_batch nc_batch;
+typedef struct nc_graph nc_graph;
+
+/* Resident per-degree plans for one modulus q (L limbs), created on first use. */
+int nc_batch_create(uint32_t L, const uint32_t* q, nc_batch** out);
+/* Destroys the plans; every graph captured from this batch must be gone. */
+void nc_batch_destroy(nc_batch* batch);
+/* The plan of degree N (created if needed); owned by the batch. */
+int nc_batch_plan(nc_batch* batch, uint32_t N, nc_plan** out);
+/* Autotunes each group's (N, count) on `stream` (nc_plan_autotune). */
+int nc_batch_autotune(nc_batch* batch, const nc_group* groups, uint32_t n, void* stream);
+/* Enqueues every group on `stream`, one after another, without a graph. */
+int nc_batch_run(nc_batch* batch, const nc_group* groups, uint32_t n, void* stream);
+/* Captures every group's pipeline into one executable HIP graph: one branch
+ * per distinct degree (branches run concurrently; groups of one degree run in
+ * order within its branch), each with the configuration its plan has for that
+ * count, and workspace owned by the graph.  The graph is bound to the
+ * groups' buffers; replays recompute from whatever they then hold. */
+int nc_batch_capture(nc_batch* batch, const nc_group* groups, uint32_t n, nc_graph** out);
+/* One launch of the whole graph on `stream` (returns without waiting). */
+int nc_graph_launch(nc_graph* graph, void* stream);
+void nc_graph_destroy(nc_graph* graph);
+
+/* ---- serving -------------------------------------------------------------
+ * A request is one product of any served degree, with operands and result in
+ * host memory.  A scheduler thread batches waiting requests per degree and
+ * keeps up to `lanes_per_degree` batches of each degree in flight on their
+ * own streams (continuous batching: requests that arrive while others run
+ * join the next batch as lanes free up); each batch's copies in and out
+ * overlap other batches' kernels.  A request is admitted only while the
+ * device bytes of admitted, unfinished requests (its footprint: operands,
+ * result and residue workspace, nc_server_footprint) stay within
+ * hbm_budget.  Host buffers from nc_host_alloc (pinned) let copies overlap;
+ * other host memory works but copies synchronously. */
+typedef struct nc_server_config {
+    uint32_t L;
+    const uint32_t* q;
+    const uint32_t* degrees;   /* ring degrees served (tables resident)    */
+    uint32_t n_degrees;
+    uint32_t max_batch;        /* products per launch (per lane)            */
+    uint32_t lanes_per_degree; /* batches of one degree in flight at once   */
+    uint64_t hbm_budget;       /* device bytes of admitted requests          */
+    uint32_t autotune;         /* 1: autotune each degree at creation         */
+} nc_server_config;
+
+typedef struct nc_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t launches;           /* batches run                              */
+    uint64_t max_batch_seen;     /* largest batch launched                   */
+    uint64_t peak_bytes;         /* most device bytes admitted at once       */
+    uint64_t admission_waits;    /* times a waiting request did not fit      */
+    uint64_t device_bytes;       /* device memory the server holds           */
+} nc_server_stats;
+
+typedef struct nc_server nc_server;
+
+int nc_server_create(const nc_server_config* config, nc_server** out);
+/* Waits for every submitted request, then frees everything (null safe). */
+void nc_server_destroy(nc_server* server);
+/* Device bytes one request of degree N counts against hbm_budget. */
+int nc_server_footprint(const nc_server* server, uint32_t N, uint64_t* bytes);
+/* Queues one product (returns at once).  a, b (N x L u32) are read and c is
+ * written by the device at some later time: keep them alive and untouched
+ * until nc_server_wait reports completion. */
+int nc_server_submit(nc_server* server, uint32_t N, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                     uint64_t* ticket);
+/* Waits for one ticket (timeout_ms < 0: no limit; NC_ERR_STATE on timeout or
+ * an unknown ticket); latency_s (nullable) receives completion - submission.
+ * A ticket can be waited for once. */
+int nc_server_wait(nc_server* server, uint64_t ticket, int timeout_ms, double* latency_s);
+int nc_server_get_stats(const nc_server* server, nc_server_stats* out);
+/* Open-loop replay: request i is submitted at arrival_s[i] seconds after the
+ * call starts (arrival_s ascending); latency_s[i] receives its completion
+ * time minus its scheduled arrival, makespan_s (nullable) the time from start
+ * to the last completion. */
+int nc_server_replay(nc_server* server, uint32_t n, const uint32_t* degrees, const double* arrival_s,
+                     const uint32_t* const* a, const uint32_t* const* b, uint32_t* const* c, double* latency_s,
+                     double* makespan_s);
+
+/* Pinned host memory (hipHostMalloc) for request buffers; NULL on failure. */
+void* nc_host_alloc(size_t bytes);
+void nc_host_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEGACYCLIC_BATCH_H */

@@ -110,3 +110,22 @@ def test_check_tables_rejects():
     assert e.value.status == -1
     with pytest.raises(ncmul.NcError):
         ncmul.check_tables(1024, np.full(32, 0xFFFFFFFF, dtype=np.uint32))
+
+
+def test_batch_and_server_argument_checks_need_no_device():
+    """Argument validation of the batch / server ABI fails before any device call."""
+    lib = ncmul.lib()
+    h = ctypes.c_void_p()
+    q = modulus("q868")
+    p32 = ctypes.POINTER(ctypes.c_uint32)
+    assert lib.nc_batch_create(28, None, ctypes.byref(h)) == -1
+    assert lib.nc_batch_create(0, q.ctypes.data_as(p32), ctypes.byref(h)) == -1
+    assert lib.nc_batch_create(28, q.ctypes.data_as(p32), None) == -1
+    assert lib.nc_server_create(None, ctypes.byref(h)) == -1
+    cfg = ncmul.ServerConfig(28, q.ctypes.data_as(p32), None, 0, 8, 2, 1 << 30, 0)
+    assert lib.nc_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert not h.value
+    for f in (lib.nc_batch_destroy, lib.nc_graph_destroy, lib.nc_server_destroy, lib.nc_host_free):
+        f(None)  # null safe
+    assert lib.nc_graph_launch(None, None) == -1
+    assert lib.nc_server_submit(None, 1024, None, None, None, None) == -1

@@ -1,0 +1,207 @@
+"""GPU tests of the batched and served forms (include/negacyclic_batch.h):
+every product they return must equal the plain path's product of the same
+operands, bit for bit, and the plain path is itself pinned to the oracle and
+the golden vectors (test_gpu_parity.py); here a few products are also checked
+against the oracle directly.
+
+  * mixed-degree batch: 256 products, N log-uniform in {2^10, ..., 2^15},
+    captured as one HIP graph; every replay (with eager work and
+    synchronisation between replays, and with new operands written into the
+    bound buffers) equals the plain path;
+  * serving: requests of mixed degree submitted together (continuous
+    batching, overlapping lanes) return what each returns alone; a tight HBM
+    budget forces admission waits without changing results; an open-loop
+    Poisson replay completes every request correctly.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import ncmul
+import oracle as O
+from conftest import modulus
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def mixed_degrees(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    return [1 << int(x) for x in rng.integers(lo, hi + 1, size=n)]
+
+
+def make_groups(degrees, q, seed0):
+    """Group products by degree; returns (groups, host operands per product)."""
+    L = q.shape[0]
+    by = {}
+    for i, N in enumerate(degrees):
+        by.setdefault(N, []).append(i)
+    groups, host = [], {}
+    for N, idx in sorted(by.items()):
+        a = np.stack([ncmul.gen_uniform(2 * (seed0 + i), N, q) for i in idx])
+        b = np.stack([ncmul.gen_uniform(2 * (seed0 + i) + 1, N, q) for i in idx])
+        d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+        d_c = ncmul.empty_device_u32((len(idx), N, L))
+        groups.append((N, len(idx), d_a, d_b, d_c))
+        host[N] = (idx, a, b)
+    return groups, host
+
+
+def plain_products(q, host):
+    """The plain path: one product per call (nc_multiply), per degree plan."""
+    out = {}
+    for N, (idx, a, b) in host.items():
+        with ncmul.Plan(N, q) as p:
+            out[N] = np.stack([p.multiply(a[i], b[i]) for i in range(len(idx))])
+    return out
+
+
+def test_mixed_degree_graph_matches_plain(gpu, q868):
+    degrees = mixed_degrees(256, 10, 15, seed=4)
+    say("mixed batch degrees", sorted(set(degrees)))
+    groups, host = make_groups(degrees, q868, 1000)
+    want = plain_products(q868, host)
+    with ncmul.Batch(q868) as batch:
+        batch.autotune([(N, n, a, b, c) for N, n, a, b, c in groups])
+        graph = batch.capture(groups)
+        for rep in range(3):
+            for _, _, _, _, c in groups:
+                c.zero_()
+            graph.launch()
+            torch.cuda.synchronize()
+            for N, n, a, b, c in groups:
+                assert np.array_equal(ncmul.from_device_u32(c), want[N]), (rep, N)
+            # eager work on the same plans between replays
+            batch.run(groups[:2])
+            torch.cuda.synchronize()
+        # the graph is bound to the buffers: new operands give new products
+        N, n, d_a, d_b, d_c = groups[-1]
+        d_a.copy_(torch.flip(d_a, dims=[0]))
+        graph.launch()
+        torch.cuda.synchronize()
+        got, a_new, b_host = ncmul.from_device_u32(d_c), ncmul.from_device_u32(d_a), host[N][2]
+        with ncmul.Plan(N, q868) as p:
+            for i in range(n):
+                assert np.array_equal(got[i], p.multiply(a_new[i], b_host[i])), i
+        assert np.array_equal(got[0], O.Plan(N, q868).multiply(a_new[0], b_host[0]))
+
+
+def test_batch_run_matches_graph(gpu, q868):
+    degrees = mixed_degrees(24, 10, 13, seed=6)
+    groups, host = make_groups(degrees, q868, 3000)
+    with ncmul.Batch(q868) as batch:
+        batch.run(groups)
+        torch.cuda.synchronize()
+        eager = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
+        graph = batch.capture(groups)
+        for _, _, _, _, c in groups:
+            c.zero_()
+        graph.launch()
+        torch.cuda.synchronize()
+        for N, _, _, _, c in groups:
+            assert np.array_equal(ncmul.from_device_u32(c), eager[N]), N
+        N0 = groups[0][0]
+        idx, a, b = host[N0]
+        assert np.array_equal(eager[N0][0], O.Plan(N0, q868).multiply(a[0], b[0]))
+
+
+def test_batch_rejects_bad_groups(gpu, q868):
+    with ncmul.Batch(q868) as batch:
+        with pytest.raises(ncmul.NcError):
+            batch.run([])
+        d = ncmul.empty_device_u32((1, 1000, 28))
+        with pytest.raises(ncmul.NcError):
+            batch.run([(1000, 1, d, d, d)])  # not a power of two
+
+
+def _requests(degrees, q, seed0):
+    L = q.shape[0]
+    reqs = []
+    for i, N in enumerate(degrees):
+        a, b, c = ncmul.host_array((N, L)), ncmul.host_array((N, L)), ncmul.host_array((N, L))
+        a[:] = ncmul.gen_uniform(2 * (seed0 + i), N, q)
+        b[:] = ncmul.gen_uniform(2 * (seed0 + i) + 1, N, q)
+        c[:] = 0
+        reqs.append((N, a, b, c))
+    return reqs
+
+
+def _alone(q, reqs):
+    plans, out = {}, []
+    try:
+        for N, a, b, _ in reqs:
+            if N not in plans:
+                plans[N] = ncmul.Plan(N, q)
+            out.append(plans[N].multiply(a, b))
+    finally:
+        for p in plans.values():
+            p.close()
+    return out
+
+
+def test_server_mix_equals_alone(gpu, q868):
+    degrees = mixed_degrees(80, 11, 14, seed=8)
+    reqs = _requests(degrees, q868, 5000)
+    alone = _alone(q868, reqs)
+    with ncmul.Server(q868, sorted(set(degrees)), max_batch=8, lanes=2, autotune=False) as srv:
+        tickets = [srv.submit(a, b, c) for _, a, b, c in reqs]  # all in flight together
+        lat = [srv.wait(t, timeout_ms=60000) for t in tickets]
+        st = srv.stats()
+    say("server stats", st)
+    assert all(x > 0 for x in lat)
+    assert st["completed"] == len(reqs) and st["launches"] < len(reqs)  # batched, not one per request
+    for i, (N, a, b, c) in enumerate(reqs):
+        assert np.array_equal(c, alone[i]), (i, N)
+    N, a, b, c = reqs[0]
+    assert np.array_equal(c, O.Plan(N, q868).multiply(np.asarray(a), np.asarray(b)))
+
+
+def test_server_admission_budget(gpu, q868):
+    degrees = [4096] * 24 + [8192] * 8
+    reqs = _requests(degrees, q868, 7000)
+    alone = _alone(q868, reqs)
+    with ncmul.Server(q868, [4096, 8192], max_batch=8, lanes=2, hbm_budget=1 << 40, autotune=False) as probe:
+        f4, f8 = probe.footprint(4096), probe.footprint(8192)
+    budget = 2 * f8 + f4  # room for about two requests at a time
+    with ncmul.Server(q868, [4096, 8192], max_batch=8, lanes=2, hbm_budget=budget, autotune=False) as srv:
+        tickets = [srv.submit(a, b, c) for _, a, b, c in reqs]
+        for t in tickets:
+            srv.wait(t, timeout_ms=60000)
+        st = srv.stats()
+    say("admission stats", st, "budget", budget)
+    assert st["peak_bytes"] <= budget and st["admission_waits"] > 0
+    for i, (N, a, b, c) in enumerate(reqs):
+        assert np.array_equal(c, alone[i]), (i, N)
+
+
+def test_server_rejects(gpu, q868):
+    with pytest.raises(ncmul.NcError):  # budget below one request
+        ncmul.Server(q868, [1024], hbm_budget=1024, autotune=False)
+    with ncmul.Server(q868, [1024], autotune=False) as srv:
+        a = ncmul.host_array((2048, 28))
+        with pytest.raises(ncmul.NcError):  # degree not served
+            srv.submit(a, a, a)
+        with pytest.raises(ncmul.NcError):
+            srv.wait(12345, timeout_ms=0)
+
+
+def test_server_poisson_replay(gpu, q868):
+    n = 200
+    degrees = mixed_degrees(n, 11, 16, seed=10)
+    reqs = _requests(degrees, q868, 9000)
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / 2000.0, size=n))  # 2000 requests/s
+    with ncmul.Server(q868, sorted(set(degrees)), max_batch=16, lanes=2, autotune=False) as srv:
+        lat, makespan = srv.replay([r[0] for r in reqs], arrivals, [r[1] for r in reqs], [r[2] for r in reqs],
+                                   [r[3] for r in reqs])
+        st = srv.stats()
+    say("replay", n, "makespan", makespan, "p50", np.percentile(lat, 50), "p99", np.percentile(lat, 99), st)
+    assert st["completed"] == n and (lat > 0).all() and makespan >= arrivals[-1]
+    alone = _alone(q868, [reqs[i] for i in range(0, n, 10)])
+    for j, i in enumerate(range(0, n, 10)):
+        assert np.array_equal(reqs[i][3], alone[j]), i
