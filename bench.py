@@ -383,6 +383,7 @@ def main() -> int:
                         f"{B} products per step, device resident",
             "N": N, "L": L, "batch": B, "k62": batched.k62, "k30": batched.k30,
             "batched": {"prime_width": w_batch, "ntt_split": s1_batch, "mfma_paths": batched.choice(B)[2],
+                        "chunks": batched.choice(B)[3],
                         "outer_forward_fused_into_decompose": fused},
             "plain": {"prime_width": w_plain, "ntt_split": s1_plain, "mfma_paths": plain.choice(1)[2]},
         },

@@ -85,6 +85,7 @@ Choice choose(const nc_plan* p, uint32_t batch) {
     else if (t.tuned) c = t.choice;
     else c = heuristic(p, batch);
     if (p->mfma_mode >= 0) c.mfma = (uint32_t)p->mfma_mode;
+    if (p->chunks_forced) c.chunks = p->chunks_forced;
     return c;
 }
 
@@ -113,8 +114,44 @@ int check_width(uint32_t width) {
     return NC_OK;
 }
 
+// Chunked batches: chunk i runs on internal stream i % 2; the second stream
+// starts once chunk 0's decomposition is done, so one chunk's VALU-bound NTT
+// overlaps another chunk's memory-bound decomposition, outer stages and CRT.
+// Chunks use disjoint slices of the workspace and of a, b, c.
+int run_chunked(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
+                const Choice& c) {
+    const uint32_t n = std::min(c.chunks, batch);
+    const uint32_t per = (batch + n - 1) / n;
+    const size_t words = (size_t)p->host.N * p->host.L;
+    NC_HIP(hipEventRecord(p->ev_fork, s), "fork");
+    NC_HIP(hipStreamWaitEvent(p->sub[0], p->ev_fork, 0), "fork");
+    NC_HIP(hipStreamWaitEvent(p->sub[1], p->ev_fork, 0), "fork");
+    const bool dm = (c.mfma & kMfmaDecompose) != 0;
+    const bool fuse = nc::decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
+    for (uint32_t i = 0; i * per < batch; ++i) {
+        const uint32_t off = i * per, cnt = std::min(per, batch - off);
+        hipStream_t st = p->sub[i & 1];
+        void* wa = static_cast<char*>(p->d_res_a) + (size_t)off * p->res_bytes;
+        void* wb = static_cast<char*>(p->d_res_b) + (size_t)off * p->res_bytes;
+        if (i == 1) NC_HIP(hipStreamWaitEvent(st, p->ev_first, 0), "stagger");
+        NC_HIP(nc::launch_decompose(p->dev, (int)c.width, d_a + off * words, wa, cnt, d_b + off * words, wb, cnt, st,
+                                    dm, fuse ? c.log_blocks : 0u),
+               "decompose launch");
+        if (i == 0) NC_HIP(hipEventRecord(p->ev_first, st), "stagger");
+        NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, wa, wb, cnt, c.log_blocks, st, fuse), "ntt launch");
+        NC_HIP(nc::launch_crt(p->dev, (int)c.width, wa, d_c + off * words, cnt, st, (c.mfma & kMfmaCrt) != 0),
+               "crt launch");
+    }
+    for (int k = 0; k < 2; ++k) {
+        NC_HIP(hipEventRecord(p->ev_join[k], p->sub[k]), "join");
+        NC_HIP(hipStreamWaitEvent(s, p->ev_join[k], 0), "join");
+    }
+    return NC_OK;
+}
+
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
+    if (c.chunks > 1 && batch > 1) return run_chunked(p, d_a, d_b, d_c, batch, s, c);
     return nc::run_pipeline_ws(p, p->d_res_a, p->d_res_b, d_a, d_b, d_c, batch, s, c);
 }
 
@@ -180,6 +217,13 @@ void nc_plan_destroy(nc_plan* p) {
     if (p->d_io) (void)hipFree(p->d_io);
     if (p->h_pin) (void)hipHostFree(p->h_pin);
     if (p->stream) (void)hipStreamDestroy(p->stream);
+    for (int k = 0; k < 2; ++k) {
+        if (p->sub[k]) (void)hipStreamSynchronize(p->sub[k]);
+        if (p->sub[k]) (void)hipStreamDestroy(p->sub[k]);
+        if (p->ev_join[k]) (void)hipEventDestroy(p->ev_join[k]);
+    }
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_first) (void)hipEventDestroy(p->ev_first);
     delete p;
 }
 
@@ -303,6 +347,13 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
 
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hip_fail(e, "stream"));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&p->sub[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join[k], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_first, hipEventDisableTiming);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "chunk streams"));
     const size_t io = (size_t)t.N * t.L * sizeof(uint32_t);
     e = hipMalloc(&p->d_io, 3 * io);
     if (e != hipSuccess) return cleanup(hip_fail(e, "staging"));
@@ -362,14 +413,22 @@ int nc_plan_set_tuning(nc_plan* p, uint32_t width, uint32_t log_blocks) {
     return NC_OK;
 }
 
+int nc_plan_set_chunks(nc_plan* p, uint32_t chunks) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (chunks > 64) return fail(NC_ERR_INVALID, "chunks must be in [0, 64]");
+    p->chunks_forced = chunks;
+    return NC_OK;
+}
+
 int nc_plan_get_choice(const nc_plan* p, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
-                       uint32_t* mfma_paths) {
+                       uint32_t* mfma_paths, uint32_t* chunks) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
     if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
     const Choice c = choose(p, batch);
     if (width) *width = c.width;
     if (log_blocks) *log_blocks = c.log_blocks;
     if (mfma_paths) *mfma_paths = mfma_in_use(p, c);
+    if (chunks) *chunks = std::min(c.chunks, batch);
     return NC_OK;
 }
 
@@ -413,12 +472,19 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
         cands.push_back({30u, mfma_default(p)});
     }
     cands.push_back({62u, 0u});
+    std::vector<uint32_t> chunk_opts = {1u};
+    if (batch >= 8) chunk_opts.push_back(2u);
+    if (batch >= 32) chunk_opts.push_back(4u);
+    if (p->chunks_forced) chunk_opts = {p->chunks_forced};
     for (const Cand& cd : cands) {
-        for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb) {
+        for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb)
+        for (uint32_t ch : chunk_opts) {
+            if (ch > 1 && cd.width != 30 && !p->chunks_forced) continue;
             Choice c;
             c.width = cd.width;
             c.mfma = cd.mfma;
             c.log_blocks = lb;
+            c.chunks = ch;
             float ms = 0.f;
             for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {  // rep 0 warms up
                 e = hipEventRecord(t0, s);

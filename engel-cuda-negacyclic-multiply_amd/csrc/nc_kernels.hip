@@ -148,14 +148,13 @@ __global__ __launch_bounds__(256) void decompose30_kernel(const uint32_t* __rest
 }
 
 // 30-bit primes on the matrix cores: the digit sum is an int8 contraction.
-// With a_m the 4L bytes of a coefficient and t_um byte u of
-// T_m = 2^(8m) 2^32 mod p_j (table rows 4j + u),
+// With a_m the 4L bytes of a coefficient and t_um the balanced base-256
+// digits (in [-128, 127]) of T_m = 2^(8m) 2^32 mod p_j (table rows 4j + u),
 //   V_j = sum_m a_m T_m = sum_u 2^(8u) X_u,   X_u = sum_m a_m t_um.
-// The MFMA computes D_u = sum_m (a_m - 128)(t_um - 128) (signed int8 inputs,
-// exact int32 sums), and X_u = D_u + 128 Sa + 128 sum_m t_um - 4L 128^2 with
-// Sa = sum_m a_m, so V_j = sum_u 2^(8u) D_u + 128 Sa 0x01010101 + G_j, G_j
-// precomputed.  0 <= V_j < 4L 255 2^30 < p 2^32, and one REDC (R = 2^32)
-// gives the residue (the table carries the 2^32).
+// The MFMA computes D_u = sum_m (a_m - 128) t_um (signed int8 inputs, exact
+// int32 sums), X_u = D_u + 128 sum_m t_um, so V_j = sum_u 2^(8u) D_u + G_j
+// with G_j = 128 sum_m T_m precomputed.  0 <= V_j < 4L 255 2^30 < p 2^32, and
+// one REDC (R = 2^32) gives the residue (the table carries the 2^32).
 // Layout (v_mfma_i32_32x32x32_i8): the A operand is the table (rows =
 // prime x byte, 8 primes per 32-row tile), the B operand is 32 coefficients'
 // bytes, so lane l receives all four byte sums of primes 2h + l/32 (h = 0..3)
@@ -175,11 +174,11 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 // lane holds, per prime, the 2^S1 values those stages combine.  Output is
 // then lazy in [0, 4p), exactly what ntt_outer_fwd_kernel<W30, S1> leaves.
 template <int S1>
-__global__ __launch_bounds__(256) void decompose30_mfma_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void decompose30_mfma_kernel(
     const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
     uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int64_t* __restrict__ gtab,
     const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
-    uint32_t tiles) {
+    uint32_t tiles, uint32_t nb) {
     constexpr int G = 1 << S1;                 // spread tiles per column group
     constexpr int NG = kMfmaCoeffTiles / G;    // column groups per wave
     static_assert(NG >= 1, "at most 4 spread tiles");
@@ -206,36 +205,33 @@ __global__ __launch_bounds__(256) void decompose30_mfma_kernel(
     const uint32_t span = N >> S1;                      // columns of one spread tile group
     const uint32_t chunks = span / 32;                  // 32-column chunks per polynomial
     const uint32_t wgs_per_poly = (chunks + 4 * NG - 1) / (4 * NG);
-    const uint32_t poly = blockIdx.x / wgs_per_poly;
-    const uint32_t chunk0 = (blockIdx.x - poly * wgs_per_poly) * (4 * NG);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane & 31u, half = lane >> 5;
+    const uint32_t nbytes = 4 * L;
+    // persistent: the tables above are loaded once per workgroup and serve
+    // every work item it takes (work item = one polynomial's 4 NG chunks)
+    for (uint32_t item = blockIdx.x; item < wgs_per_poly * (na + nb); item += gridDim.x) {
+    const uint32_t poly = item / wgs_per_poly;
+    const uint32_t chunk0 = (item - poly * wgs_per_poly) * (4 * NG);
     const bool is_a = poly < na;
     const uint32_t pi = is_a ? poly : poly - na;
     const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
     uint32_t* dst = (is_a ? res_a : res_b) + (size_t)pi * k * N;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t col = lane & 31u, half = lane >> 5;
-    const uint32_t nbytes = 4 * L;
     for (int g = 0; g < NG; ++g) {
         const uint32_t ch = chunk0 + wave * NG + g;
-        if (ch >= chunks) break;  // wave-uniform; no barrier follows
+        if (ch >= chunks) break;  // wave-uniform; no barrier in the item loop
         const uint32_t c0 = ch * 32 + col;  // this lane's column; tile e holds coefficient c0 + e span
         nc_v4i bf[G][kMfmaK / 32];
-        int64_t sa_term[G];
 #pragma unroll
         for (int e = 0; e < G; ++e) {
             const char* cp = reinterpret_cast<const char*>(src + (size_t)(c0 + e * span) * L);
-            uint32_t sa = 0;
 #pragma unroll
             for (int ks = 0; ks < kMfmaK / 32; ++ks) {
                 const uint32_t off = 32 * ks + 16 * half;
                 nc_v4i x = {0, 0, 0, 0};
                 if (off < nbytes) x = *reinterpret_cast<const nc_v4i*>(cp + off);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) sa = __builtin_amdgcn_sad_u8((uint32_t)x[q], 0u, sa);
-                bf[e][ks] = x ^ (int)0x80808080;
+                bf[e][ks] = x ^ (int)0x80808080;  // bytes a_m - 128; padding bytes meet zero table entries
             }
-            sa += __shfl_xor(sa, 32);
-            sa_term[e] = (int64_t)sa * (128ll * 0x01010101ll);
         }
         for (uint32_t t = 0; t < tiles; ++t) {
             uint32_t v[4][G];  // [h][e]
@@ -253,10 +249,10 @@ __global__ __launch_bounds__(256) void decompose30_mfma_kernel(
                     // |D| < 2^21: the two byte pairs fit int32
                     const int32_t lo = acc[4 * h] + (int32_t)((uint32_t)acc[4 * h + 1] << 8);
                     const int32_t hi = acc[4 * h + 2] + (int32_t)((uint32_t)acc[4 * h + 3] << 8);
-                    const uint64_t u =
-                        (uint64_t)((int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sa_term[e] + cj.g);
-                    const uint32_t ul = (uint32_t)u;
-                    const uint32_t r = (uint32_t)(u >> 32) + __umulhi(ul * cj.mont_neg, cj.p) + (ul != 0);
+                    const uint64_t V = (uint64_t)((int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + cj.g);
+                    // REDC: V + m p = 0 mod 2^32, (V + m p) / 2^32 < 2p
+                    const uint32_t m = (uint32_t)V * cj.mont_neg;
+                    const uint32_t r = (uint32_t)(((uint64_t)m * cj.p + V) >> 32);
                     v[h][e] = W30::sub_if_ge(r, cj.p);
                 }
             }
@@ -274,6 +270,7 @@ __global__ __launch_bounds__(256) void decompose30_mfma_kernel(
                 }
             }
         }
+    }
     }
 }
 
@@ -1281,22 +1278,23 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
                                                                d.w62.k);
     } else if (mfma && d.w30.mfma_tab && d.N >= 32) {
         constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
-        const dim3 g(((d.N + kPerWg - 1) / kPerWg) * (na + nb));
+        const uint32_t items = ((d.N + kPerWg - 1) / kPerWg) * (na + nb);
+        const dim3 g(items < 3 * d.cus ? items : 3 * d.cus);  // persistent: three workgroups per CU
         uint32_t* ra = static_cast<uint32_t*>(res_a);
         uint32_t* rb = static_cast<uint32_t*>(res_b);
         const DevSet30& w = d.w30;
         switch (fuse_s1) {
             case 0:
                 decompose30_mfma_kernel<0><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles);
+                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             case 1:
                 decompose30_mfma_kernel<1><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles);
+                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             case 2:
                 decompose30_mfma_kernel<2><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles);
+                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             default: return hipErrorInvalidValue;
         }

@@ -64,6 +64,7 @@ struct Choice {
     uint32_t width = 62;        // prime width: 62 or 30
     uint32_t log_blocks = 0;    // NTT outer split
     uint32_t mfma = kMfmaAll;   // int8 matrix-core paths where available
+    uint32_t chunks = 1;        // batch split into this many chunks on two staggered streams
 };
 
 }  // namespace nc
@@ -85,6 +86,10 @@ struct nc_plan {
     uint32_t* d_io = nullptr;   // a | b | c, N*L u32 each
     uint32_t* h_pin = nullptr;  // pinned a | b | c
     hipStream_t stream = nullptr;
+    // chunked batches: two internal streams (default priority) and the events
+    // that fork them from, stagger them and join them back to the caller's stream
+    hipStream_t sub[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_first = nullptr, ev_join[2] = {nullptr, nullptr};
     // Autotuned choice per batch-size bucket (ceil(log2 batch), 0..31);
     // `forced` pins every bucket (nc_plan_set_tuning).
     struct Tuned {
@@ -95,4 +100,5 @@ struct nc_plan {
     bool forced = false;
     nc::Choice forced_choice;
     int mfma_mode = -1;  // -1: tuned (default all on), else a kMfma* mask (nc_plan_set_mfma)
+    uint32_t chunks_forced = 0;  // 0: tuned (default 1), else every batch uses this many chunks
 };

@@ -291,10 +291,12 @@ void build30(HostTables& t, const Big& qb) {
             pl = mulmod(pl, c.c32, p);
         }
     }
-    // int8 MFMA decomposition table: X_u = sum_m a_m t_um (byte u of
-    // T_m = 2^(8m) 2^32 mod p) is computed as D_u = sum_m (a_m - 128)(t_um - 128),
-    // so X_u = D_u + 128 Sa + 128 sum_m t_um - 4L 128^2 (Sa = sum_m a_m), and
-    // sum_u X_u 2^(8u) = sum_u D_u 2^(8u) + 128 Sa 0x01010101 + G_j.
+    // int8 MFMA decomposition table: T_m = 2^(8m) 2^32 mod p_j written in
+    // balanced base-256 digits t_um in [-128, 127] (sum_u 2^(8u) t_um = T_m),
+    // which are valid signed int8 as they are.  The kernel contracts them with
+    // the coefficient bytes offset to signed (a_m - 128):
+    //   D_u = sum_m (a_m - 128) t_um,  X_u = sum_m a_m t_um = D_u + 128 sum_m t_um,
+    // so V_j = sum_m a_m T_m = sum_u 2^(8u) D_u + G_j, G_j = 128 sum_m T_m.
     if (t.L % 4 == 0) {
         const uint32_t tiles = (k + 7) / 8, nbytes = 4 * t.L;
         s.mfma_tab.assign((size_t)tiles * 32 * kMfmaK, 0);
@@ -302,17 +304,19 @@ void build30(HostTables& t, const Big& qb) {
         for (uint32_t j = 0; j < k; ++j) {
             const uint64_t p = s.primes[j];
             uint64_t place = (1ull << 32) % p;  // 2^(8m) 2^32 mod p, m = 0
-            int64_t col[4] = {0, 0, 0, 0};
+            int64_t g = 0;
             for (uint32_t m = 0; m < nbytes; ++m) {
+                int64_t rest = (int64_t)place;
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t byte = (uint32_t)(place >> (8 * u)) & 0xffu;
-                    col[u] += byte;
-                    s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m] = (int8_t)((int)byte - 128);
+                    int64_t d = rest & 0xff;
+                    if (d >= 128) d -= 256;
+                    rest = (rest - d) >> 8;
+                    s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m] = (int8_t)d;
                 }
+                if (rest != 0) throw std::runtime_error("MFMA table digit overflow");
+                g += 128 * (int64_t)place;
                 place = mulmod(place, 256, p);
             }
-            int64_t g = 0;
-            for (int u = 0; u < 4; ++u) g += (128 * col[u] - (int64_t)nbytes * 16384) * ((int64_t)1 << (8 * u));
             s.mfma_g[j] = g;
         }
     }
@@ -352,6 +356,38 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
 }
 
 namespace {
+
+// Host emulation of the int8 MFMA decomposition (decompose30_mfma_kernel):
+// for extreme and random coefficients, sum_u 2^(8u) D_u + G_j followed by the
+// kernel's REDC must equal the coefficient mod p_j.
+void check_decompose_mfma(const HostTables& t, std::mt19937_64& rng) {
+    const HostSet30& s = t.s30;
+    const uint32_t L = t.L, nbytes = 4 * L;
+    std::vector<uint32_t> limbs(L);
+    for (int trial = 0; trial < 4; ++trial) {
+        for (uint32_t l = 0; l < L; ++l) limbs[l] = trial == 0 ? 0xffffffffu : trial == 1 ? 0u : (uint32_t)rng();
+        for (uint32_t j = 0; j < s.primes.size(); ++j) {
+            const uint64_t p = s.primes[j];
+            int64_t v = s.mfma_g[j];
+            for (int u = 0; u < 4; ++u) {
+                int64_t d = 0;
+                for (uint32_t m = 0; m < (uint32_t)kMfmaK; ++m) {
+                    const int a = m < nbytes ? (int)((limbs[m / 4] >> (8 * (m % 4))) & 0xffu) : 0;
+                    d += (int64_t)(a - 128) * s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m];
+                }
+                v += d * ((int64_t)1 << (8 * u));
+            }
+            const uint64_t V = (uint64_t)v;
+            const uint32_t mneg = s.pc[j].mont_neg;
+            const uint32_t m32 = (uint32_t)V * mneg;
+            uint64_t r = (uint64_t)(((u128)m32 * p + V) >> 32);
+            if (r >= p) r -= p;
+            uint64_t horner = 0;
+            for (int l = (int)L - 1; l >= 0; --l) horner = (uint64_t)((((u128)horner) << 32 | limbs[l]) % p);
+            if (v < 0 || r != horner) throw std::runtime_error("self test: MFMA decomposition table");
+        }
+    }
+}
 
 // Host emulation of the int8 MFMA CRT contraction (crt30_mfma_kernel) on
 // extreme and random CRT-weighted residues: the q limbs must agree with
@@ -489,6 +525,7 @@ void self_test(const HostTables& t) {
         for (int l = (int)t.L - 1; l >= 0; --l) horner = (uint64_t)((((u128)horner) << 32 | limbs[l]) % p);
         if (redc != horner) throw std::runtime_error("self test: digit sum does not match the modulus");
     }
+    if (!t.s30.mfma_tab.empty()) check_decompose_mfma(t, rng);
     if (!t.s30.crt_tab.empty()) check_crt_mfma(t, rng);
 }
 

@@ -86,7 +86,8 @@ def lib():
         L.nc_plan_primes.argtypes = [_vp, ctypes.c_uint32, _u64p]
         L.nc_plan_set_tuning.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
         L.nc_plan_set_mfma.argtypes = [_vp, ctypes.c_int]
-        L.nc_plan_get_choice.argtypes = [_vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 3
+        L.nc_plan_get_choice.argtypes = [_vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 4
+        L.nc_plan_set_chunks.argtypes = [_vp, ctypes.c_uint32]
         L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
         L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
@@ -216,11 +217,15 @@ class Plan:
         self.info = self.get_info()
 
     def choice(self, batch: int):
-        """(prime width, NTT split, matrix-core paths mask) used for `batch` products."""
-        w, lb, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        _check(lib().nc_plan_get_choice(self._h, batch, ctypes.byref(w), ctypes.byref(lb), ctypes.byref(m)),
-               "nc_plan_get_choice")
-        return int(w.value), int(lb.value), int(m.value)
+        """(prime width, NTT split, matrix-core paths mask, chunks) used for `batch` products."""
+        w, lb, m, ch = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().nc_plan_get_choice(self._h, batch, ctypes.byref(w), ctypes.byref(lb), ctypes.byref(m),
+                                        ctypes.byref(ch)), "nc_plan_get_choice")
+        return int(w.value), int(lb.value), int(m.value), int(ch.value)
+
+    def set_chunks(self, chunks: int):
+        """Batch slices on two staggered streams: 0 tuned, else forced."""
+        _check(lib().nc_plan_set_chunks(self._h, int(chunks)), "nc_plan_set_chunks")
 
     def set_mfma(self, mode: int):
         """Matrix-core paths: a mask of MFMA_DECOMPOSE | MFMA_CRT (0: VALU only), or -1 tuned."""

@@ -89,9 +89,17 @@ int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
 int nc_plan_set_tuning(nc_plan* plan, uint32_t width, uint32_t log_blocks);
 
 /* The configuration calls with `batch` products use now (tuned, forced or
- * default): prime width, NTT outer split, matrix-core paths (NC_MFMA_*). */
+ * default): prime width, NTT outer split, matrix-core paths (NC_MFMA_*),
+ * chunks (any out pointer may be NULL). */
 int nc_plan_get_choice(const nc_plan* plan, uint32_t batch, uint32_t* width, uint32_t* log_blocks,
-                       uint32_t* mfma_paths);
+                       uint32_t* mfma_paths, uint32_t* chunks);
+
+/* Batches of nc_multiply_device run as `chunks` slices on two internal
+ * streams, the second started one decomposition later, so that one slice's
+ * NTT (VALU-bound) overlaps another's decomposition and CRT (memory-bound).
+ * chunks 0 lets nc_plan_autotune decide (default 1: no split); 1..64 pins it.
+ * Output bits never depend on it. */
+int nc_plan_set_chunks(nc_plan* plan, uint32_t chunks);
 
 /* Matrix-core (int8 MFMA) paths of the 30-bit prime set:
  *   NC_MFMA_DECOMPOSE  limbs -> residues (L % 4 == 0, N >= 32)
@@ -105,7 +113,7 @@ int nc_plan_get_choice(const nc_plan* plan, uint32_t batch, uint32_t* width, uin
 #define NC_MFMA_CRT 2
 int nc_plan_set_mfma(nc_plan* plan, int mode);
 
-/* Autotuner: times every (prime width, NTT split) for `batch` products on
+/* Autotuner: times every (prime width, NTT split, matrix-core paths, chunks) for `batch` products on
  * `stream` (zero inputs; the kernels' work does not depend on the data) and
  * keeps the fastest for batches of that size bucket (ceil(log2 batch)).
  * Clears a setting forced by nc_plan_set_tuning.  Output bits never depend

@@ -207,6 +207,30 @@ def test_device_batch(gpu, N, width, q868):
             assert np.array_equal(got2[i], got[i])
 
 
+@pytest.mark.parametrize("width", WIDTHS)
+@pytest.mark.parametrize("chunks", [2, 3, 8])
+def test_chunked_batch(gpu, width, chunks, q868):
+    """Batches split into chunks on two staggered streams (uneven last chunk,
+    more chunks than products) give the same bits as the oracle."""
+    N, B = 1024, 7
+    say("chunked", width, chunks)
+    a = np.stack([ncmul.gen_uniform(300 + 2 * i, N, q868) for i in range(B)])
+    b = np.stack([ncmul.gen_uniform(301 + 2 * i, N, q868) for i in range(B)])
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_tuning(width)
+        plan.set_chunks(chunks)
+        assert plan.choice(B)[3] == min(chunks, B)
+        d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+        d_c = ncmul.empty_device_u32((B, N, 28))
+        for rep in range(2):
+            d_c.zero_()
+            plan.multiply_device(d_a, d_b, d_c, B)
+            got = ncmul.from_device_u32(d_c)
+            for i in range(B):
+                assert np.array_equal(got[i], orc.multiply(a[i], b[i])), (rep, i)
+
+
 def test_autotune_keeps_bits(gpu, q868):
     N = 4096
     a, b = ncmul.gen_uniform(61, N, q868), ncmul.gen_uniform(62, N, q868)
