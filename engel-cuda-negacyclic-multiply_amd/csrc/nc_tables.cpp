@@ -273,7 +273,8 @@ void build30(HostTables& t, const Big& qb) {
         r.nr = mulmod(r.nr, y[j], p);
         r.wr = mulmod(r.wr, y[j], p);
         for (uint32_t x = 0; x < N; ++x) {
-            s.tw_fwd[(size_t)j * N + x] = Tw32{(uint32_t)r.fwd[x], x ? (uint32_t)shoup_w(r.fwd[x], p, 32) : 0u};
+            // forward twiddles stored negated (2^32 - w) for the one-multiply-add butterfly
+            s.tw_fwd[(size_t)j * N + x] = Tw32{0u - (uint32_t)r.fwd[x], x ? (uint32_t)shoup_w(r.fwd[x], p, 32) : 0u};
             s.tw_inv[(size_t)j * N + x] = Tw32{(uint32_t)r.inv[x], x ? (uint32_t)shoup_w(r.inv[x], p, 32) : 0u};
         }
         PrimeConst32& c = s.pc[j];
@@ -502,7 +503,7 @@ void self_test(const HostTables& t) {
     for (uint32_t j = 0; j < t.s30.primes.size(); ++j) {
         const PrimeConst32& c = t.s30.pc[j];
         const uint32_t p = c.p;
-        check_prime(p, t.s30.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s30.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 30);
+        check_prime(p, 0u - t.s30.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s30.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 30);
         if ((uint32_t)(p * (0u - c.mont_neg)) != 1u) throw std::runtime_error("self test: Montgomery constant");
         for (int trial = 0; trial < 16; ++trial) {
             const uint32_t a = (uint32_t)rng(), w = (uint32_t)(rng() % p), ws = (uint32_t)shoup_w(w, p, 32);
