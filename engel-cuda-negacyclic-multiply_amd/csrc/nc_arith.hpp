@@ -54,16 +54,6 @@ struct W30 {
     }
 };
 
-// lo32(a b + c) as one v_mad_u64_u32 (the compiler would otherwise split a
-// low-half multiply-add into v_mul_lo_u32 + v_add_u32).  Only the low half of
-// c reaches the low half of the result, so c can be any 64-bit value with the
-// wanted low half, such as a full product already held in a register pair.
-__device__ __forceinline__ uint32_t mad_lo32(uint32_t a, uint32_t b, uint64_t c) {
-    uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
-    return (uint32_t)r;
-}
-
 // Harvey butterflies on a policy A.
 template <class A>
 struct Bfly {
@@ -91,35 +81,32 @@ struct Bfly {
     }
 };
 
-// 30-bit butterflies with one multiply-add per twiddle product.  Forward
-// twiddles are stored negated (w- = 2^32 - w, quotient ws of w): with
-// q = hi(y ws), nv = lo(q p + y w-) = -(y w - q p) = -v mod 2^32, v in [0, 2p)
-// (Shoup), so x' = u - nv = u + v and y' = u + nv + 2p = u - v + 2p, both
-// exact in [0, 4p): 7 instructions instead of 9.  Inverse: v = lo(q (2^32 - p)
-// + d w) = d w - q p.
+// 30-bit butterflies.  Forward twiddles are stored negated (w- = 2^32 - w,
+// quotient ws of w): with q = hi(y ws), nv = q p + y w- = -(y w - q p) = -v
+// mod 2^32, v in [0, 2p) (Shoup), so x' = u - nv = u + v and
+// y' = u + nv + 2p = u - v + 2p (one three-operand add), both exact in
+// [0, 4p): 8 instructions instead of 9.
 template <>
 struct Bfly<W30> {
     using T = uint32_t;
     __device__ __forceinline__ static void ct(T& x, T& y, Tw32 t, T p, T p2) {
         const T u = W30::sub_if_ge(x, p2);
-        const T nv = mad_lo32(__umulhi(y, t.ws), p, (uint64_t)y * t.w);
+        const T nv = __umulhi(y, t.ws) * p + y * t.w;
         x = u - nv;
         y = u + nv + p2;
     }
-    __device__ __forceinline__ static T shoup_mad(T a, T w, T ws, T p) {
-        return mad_lo32(__umulhi(a, ws), 0u - p, (uint64_t)a * w);
-    }
+    __device__ __forceinline__ static T shoup(T a, T w, T ws, T p) { return a * w - __umulhi(a, ws) * p; }
     __device__ __forceinline__ static void gs(T& x, T& y, Tw32 t, T p, T p2) {
         const T u = x + y;
         const T d = x - y + p2;
         x = W30::sub_if_ge(u, p2);
-        y = shoup_mad(d, t.w, t.ws, p);
+        y = shoup(d, t.w, t.ws, p);
     }
     __device__ __forceinline__ static void gs_last(T& x, T& y, const PrimeConst32& c) {
         const T u = x + y;
         const T d = x - y + c.p2;
-        x = shoup_mad(u, c.nr, c.nr_s, c.p);
-        y = shoup_mad(d, c.wr, c.wr_s, c.p);
+        x = shoup(u, c.nr, c.nr_s, c.p);
+        y = shoup(d, c.wr, c.wr_s, c.p);
     }
 };
 
