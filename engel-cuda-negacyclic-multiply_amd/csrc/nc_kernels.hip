@@ -509,6 +509,45 @@ __device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A
     }
 }
 
+// Lane-contiguous layout x = tid + e TPB: global loads and stores in it are
+// fully coalesced.  Pass 0's layout is lane-contiguous only when its pass has
+// 4 bits (R0 = 4); otherwise the kernel loads and stores in this layout and
+// converts through one more LDS exchange.
+template <int LOGB>
+struct LinearMap {
+    using S = InnerShape<LOGB>;
+    __device__ __forceinline__ static uint32_t base(uint32_t tid) { return tid; }
+    static constexpr uint32_t delta(int e) { return (uint32_t)e * S::TPB; }
+};
+
+template <class F, class G>
+__device__ __forceinline__ void exchange_pair_maps(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid) {
+    uint2* w = lds + lds_pad(F::base(tid));
+    uint2* r = lds + lds_pad(G::base(tid));
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = make_uint2(va[e], vb[e]);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint2 x = r[lds_pad(G::delta(e))];
+        va[e] = x.x;
+        vb[e] = x.y;
+    }
+}
+
+template <class F, class G>
+__device__ __forceinline__ void exchange_maps(uint32_t (&v)[16], uint32_t* lds, uint32_t tid) {
+    uint32_t* w = lds + lds_pad(F::base(tid));
+    uint32_t* r = lds + lds_pad(G::base(tid));
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = v[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = r[lds_pad(G::delta(e))];
+}
+
 template <int LOGB, int FROM, int TO>
 __device__ __forceinline__ void exchange_pair(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid) {
     using F = PassMap<LOGB, FROM>;
@@ -565,22 +604,26 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     const Tw32* ti = twi + (size_t)j * N;
     const bool final_out = (nblk_log == 0);
     using P0 = PassMap<LOGB, 0>;
-    T* ga0 = ra + off + P0::base(tid);
-    const T* gb0 = rb + off + P0::base(tid);
+    constexpr bool kLinear = S::R0 < 4;  // pass 0 not lane-contiguous: go through LinearMap
+    using IO = typename std::conditional<kLinear, LinearMap<LOGB>, P0>::type;
+    T* ga0 = ra + off + IO::base(tid);
+    const T* gb0 = rb + off + IO::base(tid);
     T va[16], vb[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        va[e] = active ? ga0[P0::delta(e)] : T(0);
-        vb[e] = active ? gb0[P0::delta(e)] : T(0);
+        va[e] = active ? ga0[IO::delta(e)] : T(0);
+        vb[e] = active ? gb0[IO::delta(e)] : T(0);
     }
+    if constexpr (kLinear) exchange_pair_maps<LinearMap<LOGB>, P0>(va, vb, lds2, tid);
     forward_pair_from<LOGB, 0>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2);
 #pragma unroll
     for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
     __syncthreads();  // the paired region is reused below with the one-operand layout
     inverse_from<W30, LOGB, LAST>(va, lds1, tid, gblk, (int)log_n, ti, c);
+    if constexpr (kLinear) exchange_maps<P0, LinearMap<LOGB>>(va, lds1, tid);
     if (!active) return;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) ga0[P0::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
+    for (int e = 0; e < 16; ++e) ga0[IO::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
 }
 
 // ---------------------------------------------------------------------------
