@@ -253,7 +253,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                     // REDC: V + m p = 0 mod 2^32, (V + m p) / 2^32 < 2p
                     const uint32_t m = (uint32_t)V * cj.mont_neg;
                     const uint32_t r = (uint32_t)(((uint64_t)m * cj.p + V) >> 32);
-                    v[h][e] = W30::sub_if_ge(r, cj.p);
+                    // canonical for plain residues; the fused CT stages take [0, 4p) as is
+                    v[h][e] = S1 > 0 ? r : W30::sub_if_ge(r, cj.p);
                 }
             }
 #pragma unroll
