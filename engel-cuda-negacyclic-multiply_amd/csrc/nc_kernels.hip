@@ -968,11 +968,12 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
 }
 
 // x -> x 2^-96 mod q for x < q 2^96 (NA limbs), three word-level Montgomery
-// steps and one conditional subtraction; r receives the LQ limbs of the
-// result (zero beyond L).
+// steps and one conditional subtraction, in place: the LQ limbs of the
+// result (zero beyond L) end in x[3 .. 3 + LQ).  The subtraction runs as a
+// borrow-only pass (r >= q ?) and a masked pass, so no second limb array
+// is live.
 template <int LQ, int NA>
-__device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv,
-                                       uint32_t (&r)[LQ]) {
+__device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv) {
     static_assert(NA >= LQ + 4, "room for the three steps");
 #pragma unroll
     for (int st = 0; st < 3; ++st) {
@@ -985,17 +986,20 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
             carry = (uint32_t)(t >> 32);
         }
     }
-    // r = x[3 ..] < 2q: subtract q once if r >= q
-    uint32_t d[LQ + 1];
-    uint32_t br = 0;
+    uint32_t br = 0;  // r = x[3 ..] < 2q: borrow of r - q
 #pragma unroll
     for (int l = 0; l <= LQ; ++l) {
         const uint64_t t = (uint64_t)x[3 + l] - (l < LQ ? qc->q[l] : 0u) - br;
-        d[l] = (uint32_t)t;
         br = (uint32_t)(t >> 63);
     }
+    const uint32_t mask = br ? 0u : 0xffffffffu;  // subtract q iff r >= q
+    br = 0;
 #pragma unroll
-    for (int l = 0; l < LQ; ++l) r[l] = br ? x[3 + l] : d[l];
+    for (int l = 0; l < LQ; ++l) {
+        const uint64_t t = (uint64_t)x[3 + l] - (qc->q[l] & mask) - br;
+        x[3 + l] = (uint32_t)t;
+        br = (uint32_t)(t >> 63);
+    }
 }
 
 // 30-bit primes on the matrix cores.  The weighted sum is an int8
@@ -1041,12 +1045,11 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
     const uint32_t col = lane & 31u, half = lane >> 5;
     const int64_t sb_off = 128ll * kbytes;
     const uint32_t q_inv = mc->q_inv;
-    for (uint32_t grp = blockIdx.x; grp * 256u < total; grp += gridDim.x) {
-        const uint32_t g0 = grp * 256u + wave * 64u;
-        if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
-        // B operands of both tiles, every load issued before the first use;
-        // rows j >= k read row k - 1 and are replaced by 0
-        nc_v4i bf[2][KCM];
+    // B operands of both tiles of a wave's 64 coefficients starting at g0,
+    // every load issued before the first use; rows j >= k read row k - 1 and
+    // are replaced by 0 at use
+    nc_v4i bf[2][KCM];
+    auto load_group = [&](uint32_t g0) {
 #pragma unroll
         for (int tile = 0; tile < 2; ++tile) {
             const uint32_t t0 = g0 + 32u * tile;
@@ -1066,10 +1069,17 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
                 }
             }
         }
-        int32_t dones[2];
-        int32_t elo[RT][4], ehi[RT][4], olo[RT][4], ohi[RT][4];
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
+    };
+    for (uint32_t grp = blockIdx.x; grp * 256u < total; grp += gridDim.x) {
+        const uint32_t g0 = grp * 256u + wave * 64u;
+        if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
+        load_group(g0);
+        // One row tile: both tiles' contractions (the A fragment serves both),
+        // then the exchange.  Lane (c, 0) keeps tile 0's even pseudo-limbs and
+        // receives its odd ones; lane (c, 1) receives tile 1's even ones and
+        // keeps the odd.  |D| <= Kdim 2^14 < 2^23: byte pairs fit int32.
+        auto row_tile = [&](int rt, int32_t (&elo)[4], int32_t (&ehi)[4], int32_t (&olo)[4], int32_t (&ohi)[4],
+                            int32_t* dones) {
             nc_v16i acc0 = {}, acc1 = {};
             const int8_t* arow = lds_tab + (rt * 32 + col) * kCrtRowStride + 16 * half;
 #pragma unroll
@@ -1086,13 +1096,10 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
                 acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b0, acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b1, acc1, 0, 0, 0);
             }
-            if (rt == RT - 1) {  // ones row of pseudo-limb LF + 2 + h (before the swap: own tile)
+            if (dones) {  // ones row of pseudo-limb LF + 2 + h, before the swap: own tile
                 dones[0] = acc0[12];
                 dones[1] = acc1[12];
             }
-            // |D| <= Kdim 2^14 < 2^23: both byte pairs fit int32 exactly.  Lane
-            // (c, 0) keeps tile 0's even pseudo-limbs and receives its odd ones;
-            // lane (c, 1) receives tile 1's even ones and keeps the odd.
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const uint32_t lo0 = (uint32_t)(acc0[4 * m] + (int32_t)((uint32_t)acc0[4 * m + 1] << 8));
@@ -1101,38 +1108,57 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
                 const uint32_t hi1 = (uint32_t)(acc1[4 * m + 2] + (int32_t)((uint32_t)acc1[4 * m + 3] << 8));
                 const auto sl = __builtin_amdgcn_permlane32_swap(lo0, lo1, false, false);
                 const auto sh = __builtin_amdgcn_permlane32_swap(hi0, hi1, false, false);
-                elo[rt][m] = (int32_t)sl[0];
-                olo[rt][m] = (int32_t)sl[1];
-                ehi[rt][m] = (int32_t)sh[0];
-                ohi[rt][m] = (int32_t)sh[1];
+                elo[m] = (int32_t)sl[0];
+                olo[m] = (int32_t)sl[1];
+                ehi[m] = (int32_t)sh[0];
+                ohi[m] = (int32_t)sh[1];
             }
-        }
+        };
+        // the last row tile first: it carries the byte-sum row, so every later
+        // tile's limbs can be carried into 32-bit limbs as soon as they exist
+        int32_t dones[2];
+        int32_t lelo[4], lehi[4], lolo[4], lohi[4];
+        row_tile(RT - 1, lelo, lehi, lolo, lohi, dones);
         const int64_t sb = (int64_t)(half ? dones[1] : dones[0]) + sb_off;
         const int64_t sbterm = 128 * sb * 0x01010101ll;
-        auto limb_sum = [&](int lam) -> int64_t {
-            const int rt = lam / 8, m = (lam % 8) / 2;
-            const int32_t lo = (lam & 1) ? olo[rt][m] : elo[rt][m];
-            const int32_t hi = (lam & 1) ? ohi[rt][m] : ehi[rt][m];
+        auto ysum = [&](int lam, int32_t lo, int32_t hi) -> int64_t {
             return (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sbterm + corr[lam];
         };
         constexpr int NA = LQ + 4;
         uint32_t x[NA];
         int64_t carry = 0;
 #pragma unroll
-        for (int lam = 0; lam < LQ; ++lam) {
-            const int64_t t = limb_sum(lam) + carry;
+        for (int rt = 0; rt < RT - 1; ++rt) {
+            int32_t elo[4], ehi[4], olo[4], ohi[4];
+            row_tile(rt, elo, ehi, olo, ohi, nullptr);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int lam = 8 * rt + i;
+                if (lam >= LQ) break;
+                const int m = i / 2;
+                const int64_t t = ysum(lam, (i & 1) ? olo[m] : elo[m], (i & 1) ? ohi[m] : ehi[m]) + carry;
+                x[lam] = (uint32_t)t;
+                carry = t >> 32;
+            }
+        }
+        auto last_sum = [&](int lam) -> int64_t {  // pseudo-limb of the last row tile
+            const int i = lam - 8 * (RT - 1), m = i / 2;
+            return ysum(lam, (i & 1) ? lolo[m] : lelo[m], (i & 1) ? lohi[m] : lehi[m]);
+        };
+#pragma unroll
+        for (int lam = 8 * (RT - 1); lam < LQ; ++lam) {
+            const int64_t t = last_sum(lam) + carry;
             x[lam] = (uint32_t)t;
             carry = t >> 32;
         }
         x[LQ] = (uint32_t)carry;  // the sum is non-negative and below 2^(32 L + 17)
 #pragma unroll
         for (int l = LQ + 1; l < NA; ++l) x[l] = 0;
-        // K = round(sum_j v_j / p_j) from F = sum_j v_j floor(2^56 / p_j)
-        const uint64_t F = (uint64_t)limb_sum(LF) + ((uint64_t)limb_sum(LF + 1) << 32);
+        const uint64_t F = (uint64_t)last_sum(LF) + ((uint64_t)last_sum(LF + 1) << 32);
         const uint32_t K = (uint32_t)(F >> 56) + (uint32_t)((F >> 55) & 1u);
         mac_limbs<NA>(x, mc->neg_m_r, LQ, K, 0);
-        uint32_t r[LQ];
-        redc96<LQ, NA>(x, qc, q_inv, r);
+        redc96<LQ, NA>(x, qc, q_inv);
+        const uint32_t* r = x + 3;
         // lane (c, h) holds coefficient g0 + 32 h + c = g0 + lane
         uint32_t* st = lds_out + wave * 64u * L;
 #pragma unroll
