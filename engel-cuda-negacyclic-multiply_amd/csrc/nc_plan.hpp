@@ -90,6 +90,9 @@ struct nc_plan {
     // that fork them from, stagger them and join them back to the caller's stream
     hipStream_t sub[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_first = nullptr, ev_join[2] = {nullptr, nullptr};
+    // host-path staging pipeline: one event per output piece
+    static constexpr int kHostPieces = 4;
+    hipEvent_t ev_piece[kHostPieces] = {};
     // Autotuned choice per batch-size bucket (ceil(log2 batch), 0..31);
     // `forced` pins every bucket (nc_plan_set_tuning).
     struct Tuned {
