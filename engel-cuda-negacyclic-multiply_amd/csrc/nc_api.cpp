@@ -289,7 +289,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     const size_t o30_pc = slot(b.pc.size() * sizeof(nc::PrimeConst32)), o30_pl = slot(b.place.size() * 4);
     const size_t o30_cc = slot(b.cc.size() * sizeof(nc::CrtConst32)), o30_mq = slot(b.miq.size() * 4);
     const size_t o30_qc = slot(sizeof(nc::QConst));
-    const size_t o30_mt = slot(b.mfma_tab.size()), o30_mg = slot(b.mfma_g.size() * sizeof(int64_t));
+    const size_t o30_mt = slot(b.mfma_tab.size()), o30_mg = slot(b.mfma_bias.size() * sizeof(int32_t));
     const size_t o30_ct = slot(b.crt_tab.size()), o30_cc2 = slot(b.crt_c.size() * sizeof(int64_t));
     const size_t o30_cm = slot(sizeof(nc::CrtMfmaConst));
     p->table_bytes = off;
@@ -303,7 +303,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         upload(base, o30_twf, b.tw_fwd), upload(base, o30_twi, b.tw_inv), upload(base, o30_pc, b.pc),
         upload(base, o30_pl, b.place),   upload(base, o30_cc, b.cc),      upload(base, o30_mq, b.miq),
         hipMemcpy(base + o30_qc, &b.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
-        upload(base, o30_mt, b.mfma_tab), upload(base, o30_mg, b.mfma_g),
+        upload(base, o30_mt, b.mfma_tab), upload(base, o30_mg, b.mfma_bias),
         upload(base, o30_ct, b.crt_tab),  upload(base, o30_cc2, b.crt_c),
         hipMemcpy(base + o30_cm, &b.crt_mc, sizeof(nc::CrtMfmaConst), hipMemcpyHostToDevice),
     };
@@ -331,7 +331,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     d.w30.qc = reinterpret_cast<const nc::QConst*>(base + o30_qc);
     if (!b.mfma_tab.empty()) {
         d.w30.mfma_tab = reinterpret_cast<const int8_t*>(base + o30_mt);
-        d.w30.mfma_g = reinterpret_cast<const int64_t*>(base + o30_mg);
+        d.w30.mfma_bias = reinterpret_cast<const int32_t*>(base + o30_mg);
         d.w30.mfma_tiles = (uint32_t)(b.mfma_tab.size() / (32 * nc::kMfmaK));
     }
     if (!b.crt_tab.empty()) {

@@ -64,9 +64,10 @@ struct Tw32 {
 };
 
 // Byte-split int8 MFMA decomposition table (30-bit primes, L % 4 == 0):
-// row 4 j + u of tile j / 8 holds byte u of (2^(8 m) 2^32 mod p_j) for
-// m < 4L, stored offset by -128 (int8), and 0 for m >= 4L; rows of primes
-// >= k are 0.  G_j corrects the offsets (see decompose30_mfma_kernel).
+// row 4 j + u of tile j / 8 holds the balanced base-256 digit u (int8) of
+// (2^(8 m) 2^32 mod p_j) for m < 4L, and 0 for m >= 4L; rows of primes
+// >= k are 0.  A per-row int32 bias undoes the kernel's -128 byte offset
+// (see decompose30_mfma_kernel).
 constexpr int kMfmaK = 128;                        // contraction length (bytes of a coefficient, padded)
 constexpr int kMfmaRowStride = kMfmaK + 16;        // LDS row pitch: conflict-free 16-byte reads
 constexpr int kMfmaTiles = (kMaxPrimes30 + 7) / 8; // 8 primes x 4 byte chunks = 32 rows per tile

@@ -151,10 +151,12 @@ __global__ __launch_bounds__(256) void decompose30_kernel(const uint32_t* __rest
 // With a_m the 4L bytes of a coefficient and t_um the balanced base-256
 // digits (in [-128, 127]) of T_m = 2^(8m) 2^32 mod p_j (table rows 4j + u),
 //   V_j = sum_m a_m T_m = sum_u 2^(8u) X_u,   X_u = sum_m a_m t_um.
-// The MFMA computes D_u = sum_m (a_m - 128) t_um (signed int8 inputs, exact
-// int32 sums), X_u = D_u + 128 sum_m t_um, so V_j = sum_u 2^(8u) D_u + G_j
-// with G_j = 128 sum_m T_m precomputed.  0 <= V_j < 4L 255 2^30 < p 2^32, and
-// one REDC (R = 2^32) gives the residue (the table carries the 2^32).
+// The MFMA computes X_u = sum_m (a_m - 128) t_um + 128 sum_m t_um (signed
+// int8 inputs, the row bias 128 sum_m t_um as accumulator input, exact int32
+// sums), so V_j = sum_u 2^(8u) X_u = lo + 2^16 hi with lo = X_0 + 2^8 X_1,
+// hi = X_2 + 2^8 X_3 (|X_u| < 2^22: both fit int32).  0 <= V_j < 4L 255 2^30
+// < p 2^32, and one REDC (R = 2^32) gives the residue (the table carries the
+// 2^32).
 // Layout (v_mfma_i32_32x32x32_i8): the A operand is the table (rows =
 // prime x byte, 8 primes per 32-row tile), the B operand is 32 coefficients'
 // bytes, so lane l receives all four byte sums of primes 2h + l/32 (h = 0..3)
@@ -176,7 +178,7 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 template <int S1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void decompose30_mfma_kernel(
     const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
-    uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int64_t* __restrict__ gtab,
+    uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int32_t* __restrict__ bias,
     const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
     uint32_t tiles, uint32_t nb) {
     constexpr int G = 1 << S1;                 // spread tiles per column group
@@ -184,18 +186,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     static_assert(NG >= 1, "at most 4 spread tiles");
     __shared__ int8_t lds_tab[kMfmaTiles * 32 * kMfmaRowStride];
     struct PrimeLds {
-        int64_t g;
         uint32_t p, mont_neg;
     };
     __shared__ PrimeLds lds_prime[kMfmaTiles * 8];
+    __shared__ __attribute__((aligned(16))) int32_t lds_bias[kMfmaTiles * 32];
     __shared__ Tw32 lds_tw[kMfmaTiles * 8][G];  // psi^bitrev(x), x = 1 .. G - 1, of each prime
     for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
         const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
         *reinterpret_cast<nc_v4i*>(lds_tab + row * kMfmaRowStride + chunk * 16) =
             *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
     }
+    for (uint32_t r = threadIdx.x; r < tiles * 32; r += 256) lds_bias[r] = bias[r];
     for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256) {
-        lds_prime[j] = j < k ? PrimeLds{gtab[j], pc[j].p, pc[j].mont_neg} : PrimeLds{0, 1u, 0u};
+        lds_prime[j] = j < k ? PrimeLds{pc[j].p, pc[j].mont_neg} : PrimeLds{1u, 0u};
         if constexpr (S1 > 0) {
 #pragma unroll
             for (int x = 1; x < G; ++x) lds_tw[j][x] = j < k ? twf[(size_t)j * N + x] : Tw32{0u, 0u};
@@ -235,21 +238,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
         }
         for (uint32_t t = 0; t < tiles; ++t) {
             uint32_t v[4][G];  // [h][e]
+            // 2^16 in a register the compiler cannot see through, so that
+            // lo + 2^16 hi is one v_mad_i64_i32 rather than 64-bit shifts and adds
+            int32_t two16;
+            asm("s_mov_b32 %0, 0x10000" : "=s"(two16));
+            // accumulator input: output row 8 h + 4 half + u of the tile is
+            // table row (prime 2 h + half, byte u)
+            nc_v16i c_in;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const nc_v4i b4 = *reinterpret_cast<const nc_v4i*>(lds_bias + t * 32 + 8 * h + 4 * half);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) c_in[4 * h + u] = b4[u];
+            }
 #pragma unroll
             for (int e = 0; e < G; ++e) {
-                nc_v16i acc = {};
                 const int8_t* arow = lds_tab + (t * 32 + col) * kMfmaRowStride + 16 * half;
+                nc_v16i acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*reinterpret_cast<const nc_v4i*>(arow), bf[e][0],
+                                                                    c_in, 0, 0, 0);
 #pragma unroll
-                for (int ks = 0; ks < kMfmaK / 32; ++ks)
+                for (int ks = 1; ks < kMfmaK / 32; ++ks)
                     acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*reinterpret_cast<const nc_v4i*>(arow + 32 * ks),
                                                                 bf[e][ks], acc, 0, 0, 0);
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     const PrimeLds cj = lds_prime[t * 8 + 2 * h + half];
-                    // |D| < 2^21: the two byte pairs fit int32
                     const int32_t lo = acc[4 * h] + (int32_t)((uint32_t)acc[4 * h + 1] << 8);
                     const int32_t hi = acc[4 * h + 2] + (int32_t)((uint32_t)acc[4 * h + 3] << 8);
-                    const uint64_t V = (uint64_t)((int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + cj.g);
+                    const uint64_t V = (uint64_t)((int64_t)hi * two16 + (int64_t)lo);
                     // REDC: V + m p = 0 mod 2^32, (V + m p) / 2^32 < 2p
                     const uint32_t m = (uint32_t)V * cj.mont_neg;
                     const uint32_t r = (uint32_t)(((uint64_t)m * cj.p + V) >> 32);
@@ -1355,15 +1371,15 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
         const DevSet30& w = d.w30;
         switch (fuse_s1) {
             case 0:
-                decompose30_mfma_kernel<0><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                decompose30_mfma_kernel<0><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
                                                              d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             case 1:
-                decompose30_mfma_kernel<1><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                decompose30_mfma_kernel<1><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
                                                              d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             case 2:
-                decompose30_mfma_kernel<2><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_g, w.pc, w.tw_fwd,
+                decompose30_mfma_kernel<2><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
                                                              d.N, d.L, w.k, w.mfma_tiles, nb);
                 break;
             default: return hipErrorInvalidValue;

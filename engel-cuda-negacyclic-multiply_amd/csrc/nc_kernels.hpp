@@ -31,9 +31,10 @@ struct DevSet30 {
     const CrtConst32* cc = nullptr;
     const uint32_t* miq = nullptr;
     const QConst* qc = nullptr;
-    // MFMA decomposition: [tiles][32][kMfmaK] int8 and G_j (int64), or null
+    // MFMA decomposition: [tiles][32][kMfmaK] int8 and the per-row bias
+    // [tiles][32] (int32), or null
     const int8_t* mfma_tab = nullptr;
-    const int64_t* mfma_g = nullptr;
+    const int32_t* mfma_bias = nullptr;
     uint32_t mfma_tiles = 0;
     // MFMA CRT: [crt_rt * 32][crt_kc * 32] int8 table and [8 crt_rt]
     // corrections, or null

@@ -25,7 +25,7 @@ struct HostSet62 {
 
 struct HostSet30 {
     std::vector<int8_t> mfma_tab;  // [tiles][32][kMfmaK], empty when L % 4 != 0
-    std::vector<int64_t> mfma_g;   // [k]
+    std::vector<int32_t> mfma_bias;  // [tiles][32]: 128 sum_m t_um of each row
     // int8 MFMA CRT: [crt_rt * 32][crt_kc * 32] table (stored offset by -128)
     // and the per-pseudo-limb corrections [8 crt_rt]; empty when the
     // modulus is too narrow for its limb count (bits(q) + 54 < 32 L)

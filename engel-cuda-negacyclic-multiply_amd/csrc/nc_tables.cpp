@@ -297,28 +297,31 @@ void build30(HostTables& t, const Big& qb) {
     // which are valid signed int8 as they are.  The kernel contracts them with
     // the coefficient bytes offset to signed (a_m - 128):
     //   D_u = sum_m (a_m - 128) t_um,  X_u = sum_m a_m t_um = D_u + 128 sum_m t_um,
-    // so V_j = sum_m a_m T_m = sum_u 2^(8u) D_u + G_j, G_j = 128 sum_m T_m.
+    // so V_j = sum_m a_m T_m = sum_u 2^(8u) X_u.  The row bias 128 sum_m t_um
+    // enters the contraction as its accumulator input, so the kernel reads
+    // X_u directly.
     if (t.L % 4 == 0) {
         const uint32_t tiles = (k + 7) / 8, nbytes = 4 * t.L;
         s.mfma_tab.assign((size_t)tiles * 32 * kMfmaK, 0);
-        s.mfma_g.assign(k, 0);
+        s.mfma_bias.assign((size_t)tiles * 32, 0);
         for (uint32_t j = 0; j < k; ++j) {
             const uint64_t p = s.primes[j];
+            const size_t row0 = (size_t)(j / 8) * 32 + 4 * (j % 8);
             uint64_t place = (1ull << 32) % p;  // 2^(8m) 2^32 mod p, m = 0
-            int64_t g = 0;
+            int64_t bias[4] = {0, 0, 0, 0};
             for (uint32_t m = 0; m < nbytes; ++m) {
                 int64_t rest = (int64_t)place;
                 for (int u = 0; u < 4; ++u) {
                     int64_t d = rest & 0xff;
                     if (d >= 128) d -= 256;
                     rest = (rest - d) >> 8;
-                    s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m] = (int8_t)d;
+                    s.mfma_tab[(row0 + u) * kMfmaK + m] = (int8_t)d;
+                    bias[u] += 128 * d;
                 }
                 if (rest != 0) throw std::runtime_error("MFMA table digit overflow");
-                g += 128 * (int64_t)place;
                 place = mulmod(place, 256, p);
             }
-            s.mfma_g[j] = g;
+            for (int u = 0; u < 4; ++u) s.mfma_bias[row0 + u] = (int32_t)bias[u];
         }
     }
     build_crt_mfma(t, qb);
@@ -359,8 +362,9 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
 namespace {
 
 // Host emulation of the int8 MFMA decomposition (decompose30_mfma_kernel):
-// for extreme and random coefficients, sum_u 2^(8u) D_u + G_j followed by the
-// kernel's REDC must equal the coefficient mod p_j.
+// for extreme and random coefficients, X_u = D_u + bias, the kernel's int32
+// byte pairs lo = X_0 + 2^8 X_1, hi = X_2 + 2^8 X_3 (which must not wrap),
+// V = lo + 2^16 hi and its REDC must equal the coefficient mod p_j.
 void check_decompose_mfma(const HostTables& t, std::mt19937_64& rng) {
     const HostSet30& s = t.s30;
     const uint32_t L = t.L, nbytes = 4 * L;
@@ -369,15 +373,19 @@ void check_decompose_mfma(const HostTables& t, std::mt19937_64& rng) {
         for (uint32_t l = 0; l < L; ++l) limbs[l] = trial == 0 ? 0xffffffffu : trial == 1 ? 0u : (uint32_t)rng();
         for (uint32_t j = 0; j < s.primes.size(); ++j) {
             const uint64_t p = s.primes[j];
-            int64_t v = s.mfma_g[j];
+            const size_t row0 = (size_t)(j / 8) * 32 + 4 * (j % 8);
+            int64_t x[4];
             for (int u = 0; u < 4; ++u) {
-                int64_t d = 0;
+                int64_t d = s.mfma_bias[row0 + u];
                 for (uint32_t m = 0; m < (uint32_t)kMfmaK; ++m) {
                     const int a = m < nbytes ? (int)((limbs[m / 4] >> (8 * (m % 4))) & 0xffu) : 0;
-                    d += (int64_t)(a - 128) * s.mfma_tab[((size_t)(j / 8) * 32 + 4 * (j % 8) + u) * kMfmaK + m];
+                    d += (int64_t)(a - 128) * s.mfma_tab[(row0 + u) * kMfmaK + m];
                 }
-                v += d * ((int64_t)1 << (8 * u));
+                x[u] = d;
             }
+            const int64_t lo = x[0] + 256 * x[1], hi = x[2] + 256 * x[3];
+            if (lo != (int32_t)lo || hi != (int32_t)hi) throw std::runtime_error("self test: MFMA decomposition range");
+            const int64_t v = lo + 65536 * hi;
             const uint64_t V = (uint64_t)v;
             const uint32_t mneg = s.pc[j].mont_neg;
             const uint32_t m32 = (uint32_t)V * mneg;
