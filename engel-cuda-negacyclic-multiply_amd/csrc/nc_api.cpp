@@ -229,10 +229,55 @@ void nc_plan_destroy(nc_plan* p) {
     delete p;
 }
 
+namespace {
+
+// Sampled entries of the device-built twiddle tables (both directions,
+// both sets: first, middle and last prime; x = 1, 2, N / 2 + 1, N - 1 and
+// pseudo-random x) against host arithmetic.
+int check_device_twiddles(const nc::HostTables& t, const char* f62, const char* i62, const char* f30,
+                          const char* i30) {
+    const uint32_t k62 = (uint32_t)t.s62.primes.size(), k30 = (uint32_t)t.s30.primes.size(), N = t.N;
+    uint64_t lcg = 0x9e3779b97f4a7c15ull;
+    for (int set = 0; set < 2; ++set) {
+        const uint32_t k = set == 0 ? k62 : k30;
+        for (uint32_t j : {0u, k / 2, k - 1}) {
+            uint32_t xs[9] = {1u % N, 2u % N, (N / 2 + 1) % N, N - 1, 0, 0, 0, 0, 0};
+            for (int r = 4; r < 9; ++r) {
+                lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+                xs[r] = (uint32_t)(lcg >> 33) & (N - 1);
+            }
+            for (uint32_t x : xs) {
+                for (int inv = 0; inv < 2; ++inv) {
+                    const size_t at = (size_t)j * N + x;
+                    bool same;
+                    hipError_t e;
+                    if (set == 0) {
+                        nc::Tw got{};
+                        e = hipMemcpy(&got, (inv ? i62 : f62) + at * sizeof(nc::Tw), sizeof(got), hipMemcpyDeviceToHost);
+                        const nc::Tw want = nc::twiddle_entry62(t, j, x, inv != 0);
+                        same = got.w == want.w && got.ws == want.ws;
+                    } else {
+                        nc::Tw32 got{};
+                        e = hipMemcpy(&got, (inv ? i30 : f30) + at * sizeof(nc::Tw32), sizeof(got),
+                                      hipMemcpyDeviceToHost);
+                        const nc::Tw32 want = nc::twiddle_entry30(t, j, x, inv != 0);
+                        same = got.w == want.w && got.ws == want.ws;
+                    }
+                    if (e != hipSuccess) return hip_fail(e, "twiddle table check");
+                    if (!same) return fail(NC_ERR_SELFTEST, "self test: device twiddle table");
+                }
+            }
+        }
+    }
+    return NC_OK;
+}
+
+}  // namespace
+
 int nc_check_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     if (!q) return fail(NC_ERR_INVALID, "q is null");
     try {
-        nc::self_test(nc::build_tables(N, L, q));
+        nc::self_test(nc::build_tables(N, L, q, /*host_twiddles=*/false));
     } catch (const std::invalid_argument& e) {
         return fail(NC_ERR_INVALID, e.what());
     } catch (const std::runtime_error& e) {
@@ -251,7 +296,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     nc_plan* p = new (std::nothrow) nc_plan();
     if (!p) return fail(NC_ERR_NOMEM, "plan allocation");
     try {
-        p->host = nc::build_tables(N, L, q);
+        p->host = nc::build_tables(N, L, q, /*host_twiddles=*/false);
         nc::self_test(p->host);
     } catch (const std::invalid_argument& e) {
         delete p;
@@ -281,11 +326,14 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         off = align_up(off + bytes, 256);
         return at;
     };
-    const size_t o62_twf = slot(a.tw_fwd.size() * sizeof(nc::Tw)), o62_twi = slot(a.tw_inv.size() * sizeof(nc::Tw));
+    const size_t k62 = a.primes.size(), k30 = b.primes.size();
+    const size_t o62_twf = slot(k62 * t.N * sizeof(nc::Tw)), o62_twi = slot(k62 * t.N * sizeof(nc::Tw));
+    const size_t o62_pow = slot(a.tw_pow.size() * sizeof(nc::Tw));
     const size_t o62_pc = slot(a.pc.size() * sizeof(nc::PrimeConst)), o62_pl = slot(a.place.size() * 8);
     const size_t o62_cc = slot(a.cc.size() * sizeof(nc::CrtConst)), o62_mq = slot(a.miq.size() * 4);
     const size_t o62_qc = slot(sizeof(nc::QConst));
-    const size_t o30_twf = slot(b.tw_fwd.size() * sizeof(nc::Tw32)), o30_twi = slot(b.tw_inv.size() * sizeof(nc::Tw32));
+    const size_t o30_twf = slot(k30 * t.N * sizeof(nc::Tw32)), o30_twi = slot(k30 * t.N * sizeof(nc::Tw32));
+    const size_t o30_pow = slot(b.tw_pow.size() * sizeof(nc::Tw32));
     const size_t o30_pc = slot(b.pc.size() * sizeof(nc::PrimeConst32)), o30_pl = slot(b.place.size() * 4);
     const size_t o30_cc = slot(b.cc.size() * sizeof(nc::CrtConst32)), o30_mq = slot(b.miq.size() * 4);
     const size_t o30_qc = slot(sizeof(nc::QConst));
@@ -297,10 +345,10 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
     char* base = static_cast<char*>(p->d_tables);
     const hipError_t ups[] = {
-        upload(base, o62_twf, a.tw_fwd), upload(base, o62_twi, a.tw_inv), upload(base, o62_pc, a.pc),
+        upload(base, o62_pow, a.tw_pow), upload(base, o62_pc, a.pc),
         upload(base, o62_pl, a.place),   upload(base, o62_cc, a.cc),      upload(base, o62_mq, a.miq),
         hipMemcpy(base + o62_qc, &a.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
-        upload(base, o30_twf, b.tw_fwd), upload(base, o30_twi, b.tw_inv), upload(base, o30_pc, b.pc),
+        upload(base, o30_pow, b.tw_pow), upload(base, o30_pc, b.pc),
         upload(base, o30_pl, b.place),   upload(base, o30_cc, b.cc),      upload(base, o30_mq, b.miq),
         hipMemcpy(base + o30_qc, &b.qc, sizeof(nc::QConst), hipMemcpyHostToDevice),
         upload(base, o30_mt, b.mfma_tab), upload(base, o30_mg, b.mfma_bias),
@@ -309,6 +357,19 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     };
     for (hipError_t u : ups)
         if (u != hipSuccess) return cleanup(hip_fail(u, "table upload"));
+    // the [k][N] twiddle tables, built in place on the device
+    e = nc::launch_twiddles62(reinterpret_cast<nc::Tw*>(base + o62_twf), reinterpret_cast<nc::Tw*>(base + o62_twi),
+                              reinterpret_cast<const nc::Tw*>(base + o62_pow),
+                              reinterpret_cast<const nc::PrimeConst*>(base + o62_pc), t.log_n, (uint32_t)k62, nullptr);
+    if (e == hipSuccess)
+        e = nc::launch_twiddles30(
+            reinterpret_cast<nc::Tw32*>(base + o30_twf), reinterpret_cast<nc::Tw32*>(base + o30_twi),
+            reinterpret_cast<const nc::Tw32*>(base + o30_pow), reinterpret_cast<const nc::PrimeConst32*>(base + o30_pc),
+            t.log_n, (uint32_t)k30, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle tables"));
+    if (int rc = check_device_twiddles(t, base + o62_twf, base + o62_twi, base + o30_twf, base + o30_twi))
+        return cleanup(rc);
     nc::DevPlan& d = p->dev;
     d.N = t.N;
     d.L = t.L;
@@ -390,6 +451,56 @@ int nc_plan_primes(const nc_plan* p, uint32_t width, uint64_t* out) {
     if (int rc = check_width(width)) return rc;
     const std::vector<uint64_t>& v = width == 62 ? p->host.s62.primes : p->host.s30.primes;
     std::memcpy(out, v.data(), v.size() * sizeof(uint64_t));
+    return NC_OK;
+}
+
+int nc_plan_copy_twiddles(const nc_plan* p, uint32_t width, int inverse, void* out, size_t bytes) {
+    if (!p || !out) return fail(NC_ERR_INVALID, "null argument");
+    if (int rc = check_width(width)) return rc;
+    const size_t n = (size_t)k_of(p, width) * p->host.N;
+    const void* src;
+    size_t need;
+    if (width == 62) {
+        src = inverse ? (const void*)p->dev.w62.tw_inv : (const void*)p->dev.w62.tw_fwd;
+        need = n * sizeof(nc::Tw);
+    } else {
+        src = inverse ? (const void*)p->dev.w30.tw_inv : (const void*)p->dev.w30.tw_fwd;
+        need = n * sizeof(nc::Tw32);
+    }
+    if (bytes != need) return fail(NC_ERR_SIZE, "bytes must equal the twiddle table size");
+    NC_HIP(hipMemcpy(out, src, need, hipMemcpyDeviceToHost), "twiddle copy");
+    return NC_OK;
+}
+
+int nc_host_twiddles(uint32_t N, uint32_t L, const uint32_t* q, uint32_t width, int inverse, void* out,
+                     size_t* bytes) {
+    if (!q || !bytes) return fail(NC_ERR_INVALID, "null argument");
+    if (int rc = check_width(width)) return rc;
+    try {
+        const nc::HostTables t = nc::build_tables(N, L, q, /*host_twiddles=*/true);
+        const void* src;
+        size_t need;
+        if (width == 62) {
+            const auto& v = inverse ? t.s62.tw_inv : t.s62.tw_fwd;
+            src = v.data();
+            need = v.size() * sizeof(nc::Tw);
+        } else {
+            const auto& v = inverse ? t.s30.tw_inv : t.s30.tw_fwd;
+            src = v.data();
+            need = v.size() * sizeof(nc::Tw32);
+        }
+        const size_t cap = *bytes;
+        *bytes = need;
+        if (!out) return NC_OK;
+        if (cap < need) return fail(NC_ERR_SIZE, "buffer smaller than the twiddle table");
+        std::memcpy(out, src, need);
+    } catch (const std::invalid_argument& e) {
+        return fail(NC_ERR_INVALID, e.what());
+    } catch (const std::runtime_error& e) {
+        return fail(NC_ERR_PRIMES, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(NC_ERR_NOMEM, "host tables");
+    }
     return NC_OK;
 }
 

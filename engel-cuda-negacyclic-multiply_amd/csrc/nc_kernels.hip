@@ -1210,6 +1210,89 @@ __global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
 }
 
 // ===========================================================================
+// Twiddle tables on the device (what fherma_init builds on the host,
+// solve.cu:1224-1257, here in the merged-psi form): entry x of prime j is
+// psi_j^(+-bitrev(x)) with its Shoup quotient, {0, 0} at x = 0.  One thread
+// per (prime, x): the power is a chain of Shoup products over the set bits
+// of bitrev(x) with the table pow[j][dir][i] = psi^(+-2^i) (lazy in [0, 2p),
+// reduced once), and the quotient floor(w 2^b / p) is exact long division.
+// ===========================================================================
+__device__ __forceinline__ uint64_t shoup_quotient64(uint64_t w, uint64_t p) {
+    uint64_t r = w, q = 0;  // w < p < 2^62: 2 r stays below 2^63
+#pragma unroll 8
+    for (int i = 0; i < 64; ++i) {
+        r <<= 1;
+        q <<= 1;
+        if (r >= p) {
+            r -= p;
+            q |= 1;
+        }
+    }
+    return q;
+}
+
+__global__ __launch_bounds__(256) void twiddles62_kernel(Tw* __restrict__ fwd, Tw* __restrict__ inv,
+                                                         const Tw* __restrict__ pow, const PrimeConst* __restrict__ pc,
+                                                         uint32_t log_n, uint32_t k) {
+    const uint32_t N = 1u << log_n;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= (uint64_t)k * N) return;
+    const uint32_t j = (uint32_t)(gid >> log_n), x = (uint32_t)gid & (N - 1);
+    Tw f{0, 0}, iv{0, 0};
+    if (x != 0) {
+        const uint64_t p = pc[j].p;
+        const uint32_t e = __builtin_bitreverse32(x) >> (32 - log_n);
+        const Tw* pf = pow + (size_t)j * 2 * log_n;
+        const Tw* pi = pf + log_n;
+        uint64_t a = 1, b = 1;
+        for (uint32_t i = 0; i < log_n; ++i) {
+            if ((e >> i) & 1u) {
+                a = W62::shoup(a, pf[i].w, pf[i].ws, p);
+                b = W62::shoup(b, pi[i].w, pi[i].ws, p);
+            }
+        }
+        a = W62::sub_if_ge(a, p);
+        b = W62::sub_if_ge(b, p);
+        f = Tw{a, shoup_quotient64(a, p)};
+        iv = Tw{b, shoup_quotient64(b, p)};
+    }
+    fwd[gid] = f;
+    inv[gid] = iv;
+}
+
+// 30-bit set: the forward value is stored negated (2^32 - w), as the
+// forward butterfly expects.
+__global__ __launch_bounds__(256) void twiddles30_kernel(Tw32* __restrict__ fwd, Tw32* __restrict__ inv,
+                                                         const Tw32* __restrict__ pow,
+                                                         const PrimeConst32* __restrict__ pc, uint32_t log_n,
+                                                         uint32_t k) {
+    const uint32_t N = 1u << log_n;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (gid >= (uint64_t)k * N) return;
+    const uint32_t j = (uint32_t)(gid >> log_n), x = (uint32_t)gid & (N - 1);
+    Tw32 f{0u, 0u}, iv{0u, 0u};
+    if (x != 0) {
+        const uint32_t p = pc[j].p;
+        const uint32_t e = __builtin_bitreverse32(x) >> (32 - log_n);
+        const Tw32* pf = pow + (size_t)j * 2 * log_n;
+        const Tw32* pi = pf + log_n;
+        uint32_t a = 1, b = 1;
+        for (uint32_t i = 0; i < log_n; ++i) {
+            if ((e >> i) & 1u) {
+                a = W30::shoup(a, pf[i].w, pf[i].ws, p);
+                b = W30::shoup(b, pi[i].w, pi[i].ws, p);
+            }
+        }
+        a = W30::sub_if_ge(a, p);
+        b = W30::sub_if_ge(b, p);
+        f = Tw32{0u - a, (uint32_t)(((uint64_t)a << 32) / p)};
+        iv = Tw32{b, (uint32_t)(((uint64_t)b << 32) / p)};
+    }
+    fwd[gid] = f;
+    inv[gid] = iv;
+}
+
+// ===========================================================================
 // 32x32->64 multiply throughput probe: eight independent v_mad_u64_u32 chains
 // per lane, no other VALU work in the loop.
 // ===========================================================================
@@ -1483,6 +1566,22 @@ hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* ou
 
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s) {
     mul32_peak_kernel<<<blocks, 256, 0, s>>>(in, out, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddles62(Tw* fwd, Tw* inv, const Tw* pow, const PrimeConst* pc, uint32_t log_n, uint32_t k,
+                             hipStream_t s) {
+    if (log_n < 1 || log_n > (uint32_t)kMaxLogN || k == 0) return hipErrorInvalidValue;
+    const uint64_t n = (uint64_t)k << log_n;
+    twiddles62_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(fwd, inv, pow, pc, log_n, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddles30(Tw32* fwd, Tw32* inv, const Tw32* pow, const PrimeConst32* pc, uint32_t log_n,
+                             uint32_t k, hipStream_t s) {
+    if (log_n < 1 || log_n > (uint32_t)kMaxLogN || k == 0) return hipErrorInvalidValue;
+    const uint64_t n = (uint64_t)k << log_n;
+    twiddles30_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(fwd, inv, pow, pc, log_n, k);
     return hipGetLastError();
 }
 

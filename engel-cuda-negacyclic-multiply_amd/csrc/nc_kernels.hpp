@@ -87,6 +87,13 @@ bool crt_mfma_available(const DevPlan& d);
 hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s,
                       bool mfma = true);
 
+// Twiddle tables [k][N] built on the device from pow [k][2][log_n]
+// (psi^(2^i), psi^-(2^i) with Shoup quotients) and the primes in pc.
+hipError_t launch_twiddles62(Tw* fwd, Tw* inv, const Tw* pow, const PrimeConst* pc, uint32_t log_n, uint32_t k,
+                             hipStream_t s);
+hipError_t launch_twiddles30(Tw32* fwd, Tw32* inv, const Tw32* pow, const PrimeConst32* pc, uint32_t log_n,
+                             uint32_t k, hipStream_t s);
+
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 

@@ -15,7 +15,8 @@ namespace nc {
 // Host copy of one prime set's tables.
 struct HostSet62 {
     std::vector<uint64_t> primes;
-    std::vector<Tw> tw_fwd, tw_inv;  // [k][N]
+    std::vector<Tw> tw_fwd, tw_inv;  // [k][N] (host-built tables only)
+    std::vector<Tw> tw_pow;          // [k][2][log_n]: psi^(2^i), psi^-(2^i) with Shoup quotients
     std::vector<PrimeConst> pc;      // [k]
     std::vector<uint64_t> place;     // [k][kMaxLimbs]
     std::vector<CrtConst> cc;        // [k]
@@ -34,7 +35,8 @@ struct HostSet30 {
     CrtMfmaConst crt_mc{};
     uint32_t crt_rt = 0, crt_kc = 0;
     std::vector<uint64_t> primes;
-    std::vector<Tw32> tw_fwd, tw_inv;
+    std::vector<Tw32> tw_fwd, tw_inv;  // [k][N] (host-built tables only; forward w stored negated)
+    std::vector<Tw32> tw_pow;          // [k][2][log_n]: psi^(2^i), psi^-(2^i) (not negated)
     std::vector<PrimeConst32> pc;
     std::vector<uint32_t> place;
     std::vector<CrtConst32> cc;
@@ -44,13 +46,23 @@ struct HostSet30 {
 
 struct HostTables {
     uint32_t N = 0, L = 0, log_n = 0, q_bits = 0;
+    // false: the [k][N] twiddle tables are left to the device builder
+    // (launch_twiddles62/30) and only tw_pow is filled
+    bool host_twiddles = true;
     HostSet62 s62;  // 62-bit primes: the reference's prime rule (solve.cu:1179-1204)
     HostSet30 s30;  // 30-bit primes: the same rule below 2^30
 };
 
-// Builds both prime sets and every table; throws std::invalid_argument /
-// std::runtime_error with a message on failure.
-HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q);
+// Builds both prime sets and every table (the [k][N] twiddles only when
+// host_twiddles); throws std::invalid_argument / std::runtime_error with a
+// message on failure.
+HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles = true);
+
+// The stored twiddle entry x of prime j (direction inverse), by host
+// arithmetic: {psi^(+-bitrev(x)), Shoup quotient}, {0, 0} at x = 0, the
+// 30-bit forward value negated.  Checks samples of device-built tables.
+Tw twiddle_entry62(const HostTables& t, uint32_t j, uint32_t x, bool inverse);
+Tw32 twiddle_entry30(const HostTables& t, uint32_t j, uint32_t x, bool inverse);
 
 // Init-time known-answer checks of both sets (the analogue of
 // solve.cu:1151-1166, 1285-1314, 1370-1415 for this formulation).

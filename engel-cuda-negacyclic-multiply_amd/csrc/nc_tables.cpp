@@ -58,31 +58,49 @@ std::vector<uint64_t> pick_primes(uint32_t N, uint32_t log_n, uint32_t q_bits, i
 // Twiddles psi^bitrev(x) / psi^-bitrev(x), x in [1, N), and the per-prime
 // values shared by both widths (as 64-bit numbers; the caller narrows).
 struct RootTables {
-    std::vector<uint64_t> fwd, inv;  // [N]
+    std::vector<uint64_t> fwd, inv;  // [N] (full tables only)
+    std::vector<uint64_t> pf, pi;    // [log_n]: psi^(2^i), psi^-(2^i)
     uint64_t nr = 0, wr = 0;         // N^-1 R mod p, psi^-bitrev(1) N^-1 R mod p
 };
 
-RootTables roots(uint64_t p, uint32_t N, uint32_t log_n, int word_bits) {
+RootTables roots(uint64_t p, uint32_t N, uint32_t log_n, int word_bits, bool full) {
     const uint64_t psi = find_psi(p, N);
     const uint64_t psi_inv = invmod(psi, p);
-    std::vector<uint64_t> pw(N), ipw(N);
-    pw[0] = ipw[0] = 1;
-    for (uint32_t i = 1; i < N; ++i) {
-        pw[i] = mulmod(pw[i - 1], psi, p);
-        ipw[i] = mulmod(ipw[i - 1], psi_inv, p);
-    }
     RootTables r;
-    r.fwd.assign(N, 0);
-    r.inv.assign(N, 0);
-    for (uint32_t x = 1; x < N; ++x) {
-        const uint32_t e = bitrev(x, log_n);
-        r.fwd[x] = pw[e];
-        r.inv[x] = ipw[e];
+    r.pf.resize(log_n);
+    r.pi.resize(log_n);
+    for (uint32_t i = 0; i < log_n; ++i) {
+        r.pf[i] = i ? mulmod(r.pf[i - 1], r.pf[i - 1], p) : psi;
+        r.pi[i] = i ? mulmod(r.pi[i - 1], r.pi[i - 1], p) : psi_inv;
+    }
+    if (full) {
+        std::vector<uint64_t> pw(N), ipw(N);
+        pw[0] = ipw[0] = 1;
+        for (uint32_t i = 1; i < N; ++i) {
+            pw[i] = mulmod(pw[i - 1], psi, p);
+            ipw[i] = mulmod(ipw[i - 1], psi_inv, p);
+        }
+        r.fwd.assign(N, 0);
+        r.inv.assign(N, 0);
+        for (uint32_t x = 1; x < N; ++x) {
+            const uint32_t e = bitrev(x, log_n);
+            r.fwd[x] = pw[e];
+            r.inv[x] = ipw[e];
+        }
     }
     const uint64_t R = (uint64_t)((((u128)1) << word_bits) % p);  // the Montgomery radix mod p
     r.nr = mulmod(invmod(N % p, p), R, p);
-    r.wr = mulmod(N > 1 ? r.inv[1] : 1, r.nr, p);
+    // psi^-bitrev(1) = psi^-(N/2)
+    r.wr = mulmod(N > 1 ? r.pi[log_n - 1] : 1, r.nr, p);
     return r;
+}
+
+// psi^(+-e) mod p from the squarings table, one factor per set bit of e
+uint64_t pow_from_table(const std::vector<uint64_t>& sq, uint32_t e, uint64_t p) {
+    uint64_t w = 1;
+    for (uint32_t i = 0; e; ++i, e >>= 1)
+        if (e & 1u) w = mulmod(w, sq[i], p);
+    return w;
 }
 
 uint64_t shoup_w(uint64_t w, uint64_t p, int word_bits) { return (uint64_t)(((u128)w << word_bits) / p); }
@@ -133,8 +151,11 @@ void build62(HostTables& t, const Big& qb) {
     Big M;
     s.primes = pick_primes(t.N, t.log_n, t.q_bits, 62, (size_t)kMaxPrimes, M);
     const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
-    s.tw_fwd.resize((size_t)k * N);
-    s.tw_inv.resize((size_t)k * N);
+    if (t.host_twiddles) {
+        s.tw_fwd.resize((size_t)k * N);
+        s.tw_inv.resize((size_t)k * N);
+    }
+    s.tw_pow.resize((size_t)k * 2 * t.log_n);
     s.pc.resize(k);
     s.place.assign((size_t)k * kMaxLimbs, 0);
     s.cc.resize(k);
@@ -145,10 +166,14 @@ void build62(HostTables& t, const Big& qb) {
     crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
-        RootTables r = roots(p, N, t.log_n, 64);
+        RootTables r = roots(p, N, t.log_n, 64, t.host_twiddles);
         r.nr = mulmod(r.nr, y[j], p);
         r.wr = mulmod(r.wr, y[j], p);
-        for (uint32_t x = 0; x < N; ++x) {
+        for (uint32_t i = 0; i < t.log_n; ++i) {
+            s.tw_pow[((size_t)j * 2 + 0) * t.log_n + i] = Tw{r.pf[i], shoup_w(r.pf[i], p, 64)};
+            s.tw_pow[((size_t)j * 2 + 1) * t.log_n + i] = Tw{r.pi[i], shoup_w(r.pi[i], p, 64)};
+        }
+        for (uint32_t x = 0; x < N && t.host_twiddles; ++x) {
             s.tw_fwd[(size_t)j * N + x] = Tw{r.fwd[x], x ? shoup_w(r.fwd[x], p, 64) : 0};
             s.tw_inv[(size_t)j * N + x] = Tw{r.inv[x], x ? shoup_w(r.inv[x], p, 64) : 0};
         }
@@ -260,8 +285,11 @@ void build30(HostTables& t, const Big& qb) {
     Big M;
     s.primes = pick_primes(t.N, t.log_n, t.q_bits, 30, (size_t)kMaxPrimes30, M);
     const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
-    s.tw_fwd.resize((size_t)k * N);
-    s.tw_inv.resize((size_t)k * N);
+    if (t.host_twiddles) {
+        s.tw_fwd.resize((size_t)k * N);
+        s.tw_inv.resize((size_t)k * N);
+    }
+    s.tw_pow.resize((size_t)k * 2 * t.log_n);
     s.pc.resize(k);
     s.place.assign((size_t)k * kMaxLimbs, 0);
     s.cc.resize(k);
@@ -269,10 +297,14 @@ void build30(HostTables& t, const Big& qb) {
     crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
-        RootTables r = roots(p, N, t.log_n, 32);
+        RootTables r = roots(p, N, t.log_n, 32, t.host_twiddles);
         r.nr = mulmod(r.nr, y[j], p);
         r.wr = mulmod(r.wr, y[j], p);
-        for (uint32_t x = 0; x < N; ++x) {
+        for (uint32_t i = 0; i < t.log_n; ++i) {
+            s.tw_pow[((size_t)j * 2 + 0) * t.log_n + i] = Tw32{(uint32_t)r.pf[i], (uint32_t)shoup_w(r.pf[i], p, 32)};
+            s.tw_pow[((size_t)j * 2 + 1) * t.log_n + i] = Tw32{(uint32_t)r.pi[i], (uint32_t)shoup_w(r.pi[i], p, 32)};
+        }
+        for (uint32_t x = 0; x < N && t.host_twiddles; ++x) {
             // forward twiddles stored negated (2^32 - w) for the one-multiply-add butterfly
             s.tw_fwd[(size_t)j * N + x] = Tw32{0u - (uint32_t)r.fwd[x], x ? (uint32_t)shoup_w(r.fwd[x], p, 32) : 0u};
             s.tw_inv[(size_t)j * N + x] = Tw32{(uint32_t)r.inv[x], x ? (uint32_t)shoup_w(r.inv[x], p, 32) : 0u};
@@ -339,7 +371,7 @@ void build30(HostTables& t, const Big& qb) {
 
 }  // namespace
 
-HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
+HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles) {
     // validation (solve.cu:1167-1172; N cap raised to 65536)
     if (N < 2 || N > (1u << kMaxLogN) || (N & (N - 1))) throw std::invalid_argument("N must be a power of two from 2 to 65536");
     if (L == 0 || L > (uint32_t)kMaxLimbs) throw std::invalid_argument("L must be in [1, 32]");
@@ -347,6 +379,7 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     HostTables t;
     t.N = N;
     t.L = L;
+    t.host_twiddles = host_twiddles;
     while ((1u << t.log_n) < N) ++t.log_n;
     Big qb(L);
     for (uint32_t l = 0; l < L; ++l) qb.v[l] = q[l];
@@ -357,6 +390,24 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q) {
     build62(t, qb);
     build30(t, qb);
     return t;
+}
+
+Tw twiddle_entry62(const HostTables& t, uint32_t j, uint32_t x, bool inverse) {
+    if (x == 0) return Tw{0, 0};
+    const uint64_t p = t.s62.primes[j];
+    std::vector<uint64_t> sq(t.log_n);
+    for (uint32_t i = 0; i < t.log_n; ++i) sq[i] = t.s62.tw_pow[((size_t)j * 2 + inverse) * t.log_n + i].w;
+    const uint64_t w = pow_from_table(sq, bitrev(x, t.log_n), p);
+    return Tw{w, shoup_w(w, p, 64)};
+}
+
+Tw32 twiddle_entry30(const HostTables& t, uint32_t j, uint32_t x, bool inverse) {
+    if (x == 0) return Tw32{0u, 0u};
+    const uint64_t p = t.s30.primes[j];
+    std::vector<uint64_t> sq(t.log_n);
+    for (uint32_t i = 0; i < t.log_n; ++i) sq[i] = t.s30.tw_pow[((size_t)j * 2 + inverse) * t.log_n + i].w;
+    const uint64_t w = pow_from_table(sq, bitrev(x, t.log_n), p);
+    return Tw32{inverse ? (uint32_t)w : 0u - (uint32_t)w, (uint32_t)shoup_w(w, p, 32)};
 }
 
 namespace {
@@ -491,7 +542,14 @@ void self_test(const HostTables& t) {
     for (uint32_t j = 0; j < t.s62.primes.size(); ++j) {
         const PrimeConst& c = t.s62.pc[j];
         const uint64_t p = c.p;
-        check_prime(p, t.s62.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s62.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 62);
+        // psi^(N/2) = psi^bitrev(1), the first stage's twiddle
+        const size_t top = t.log_n - 1;
+        check_prime(p, t.s62.tw_pow[((size_t)j * 2 + 0) * t.log_n + top].w,
+                    t.s62.tw_pow[((size_t)j * 2 + 1) * t.log_n + top].w, 62);
+        if (t.host_twiddles && t.N > 1 &&
+            (t.s62.tw_fwd[(size_t)j * t.N + 1].w != t.s62.tw_pow[((size_t)j * 2 + 0) * t.log_n + top].w ||
+             t.s62.tw_inv[(size_t)j * t.N + 1].w != t.s62.tw_pow[((size_t)j * 2 + 1) * t.log_n + top].w))
+            throw std::runtime_error("self test: twiddle table");
         if ((uint64_t)(p * (0 - c.mont_neg)) != 1) throw std::runtime_error("self test: Montgomery constant");
         for (int trial = 0; trial < 16; ++trial) {  // Shoup lazy product in [0, 2p), exact mod p
             const uint64_t a = rng(), w = rng() % p, ws = shoup_w(w, p, 64);
@@ -511,7 +569,13 @@ void self_test(const HostTables& t) {
     for (uint32_t j = 0; j < t.s30.primes.size(); ++j) {
         const PrimeConst32& c = t.s30.pc[j];
         const uint32_t p = c.p;
-        check_prime(p, 0u - t.s30.tw_fwd[(size_t)j * t.N + (t.N > 1)].w, t.s30.tw_inv[(size_t)j * t.N + (t.N > 1)].w, 30);
+        const size_t top = t.log_n - 1;
+        check_prime(p, t.s30.tw_pow[((size_t)j * 2 + 0) * t.log_n + top].w,
+                    t.s30.tw_pow[((size_t)j * 2 + 1) * t.log_n + top].w, 30);
+        if (t.host_twiddles && t.N > 1 &&
+            (0u - t.s30.tw_fwd[(size_t)j * t.N + 1].w != t.s30.tw_pow[((size_t)j * 2 + 0) * t.log_n + top].w ||
+             t.s30.tw_inv[(size_t)j * t.N + 1].w != t.s30.tw_pow[((size_t)j * 2 + 1) * t.log_n + top].w))
+            throw std::runtime_error("self test: twiddle table");
         if ((uint32_t)(p * (0u - c.mont_neg)) != 1u) throw std::runtime_error("self test: Montgomery constant");
         for (int trial = 0; trial < 16; ++trial) {
             const uint32_t a = (uint32_t)rng(), w = (uint32_t)(rng() % p), ws = (uint32_t)shoup_w(w, p, 32);

@@ -80,6 +80,9 @@ def lib():
         L.nc_last_error.argtypes = []
         L.nc_plan_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)]
         L.nc_check_tables.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        L.nc_host_twiddles.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.c_int, _vp,
+                                       ctypes.POINTER(ctypes.c_size_t)]
+        L.nc_plan_copy_twiddles.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int, _vp, ctypes.c_size_t]
         L.nc_plan_destroy.argtypes = [_vp]
         L.nc_plan_destroy.restype = None
         L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
@@ -152,6 +155,18 @@ def check_tables(N: int, q: np.ndarray):
     _check(lib().nc_check_tables(int(N), q.shape[0], _p32(q)), "nc_check_tables")
 
 
+def host_twiddles(N: int, q: np.ndarray, width: int, inverse: bool) -> np.ndarray:
+    """Twiddle table [k][N][2] ({w, Shoup quotient}) built on the host, no device."""
+    q = np.ascontiguousarray(q, dtype=np.uint32)
+    n = ctypes.c_size_t(0)
+    args = (int(N), q.shape[0], _p32(q), int(width), int(bool(inverse)))
+    _check(lib().nc_host_twiddles(*args, None, ctypes.byref(n)), "nc_host_twiddles")
+    dt = np.uint64 if width == 62 else np.uint32
+    out = np.empty(n.value // np.dtype(dt).itemsize, dtype=dt)
+    _check(lib().nc_host_twiddles(*args, _vp(out.ctypes.data), ctypes.byref(n)), "nc_host_twiddles")
+    return out.reshape(-1, N, 2)
+
+
 def measure_mul32_peak() -> float:
     """Measured 32x32->64 integer multiplies per second of the current device."""
     out = ctypes.c_double()
@@ -208,6 +223,14 @@ class Plan:
     def primes(self, width: int = 62) -> np.ndarray:
         out = np.empty(self.k(width), dtype=np.uint64)
         _check(lib().nc_plan_primes(self._h, width, out.ctypes.data_as(_u64p)), "nc_plan_primes")
+        return out
+
+    def twiddles(self, width: int, inverse: bool) -> np.ndarray:
+        """The plan's device-built twiddle table [k][N][2], copied to the host."""
+        dt = np.uint64 if width == 62 else np.uint32
+        out = np.empty((self.k(width), self.N, 2), dtype=dt)
+        _check(lib().nc_plan_copy_twiddles(self._h, int(width), int(bool(inverse)), _vp(out.ctypes.data),
+                                           out.nbytes), "nc_plan_copy_twiddles")
         return out
 
     def set_tuning(self, width: int, log_blocks=None):

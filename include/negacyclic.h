@@ -72,8 +72,11 @@ const char* nc_last_error(void);
  * (solve.cu:1150-1487): validation (1167-1172), prime rule (1174-1204),
  * twiddle / Montgomery / CRT tables (1224-1369, 1396-1398), the
  * known-answer self tests (1151-1166, 1285-1314, 1370-1415) and the
- * upload.  nc_plan_destroy replaces fherma_free (solve.cu:1530-1567) and
- * is null safe. */
+ * upload.  The [k][N] twiddle tables (the bulk of the tables, 1224-1257)
+ * are built on the device from log2(N) powers per prime, and sampled
+ * entries are checked against host arithmetic; the rest is built on the
+ * host.  nc_plan_destroy replaces fherma_free (solve.cu:1530-1567) and is
+ * null safe. */
 int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out);
 /* Builds the host tables of (N, L, q) and runs the known-answer self tests
  * without touching a device (the host half of nc_plan_create). */
@@ -82,6 +85,19 @@ void nc_plan_destroy(nc_plan* plan);
 int nc_plan_get_info(const nc_plan* plan, nc_plan_info* info);
 /* The primes of one set (width 62 or 30), host copy; out holds k values. */
 int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
+/* Copies one device twiddle table of a plan to host memory: width 62 gives
+ * k x N {w, ws} pairs of uint64 (16 bytes each), width 30 k x N pairs of
+ * uint32 (8 bytes, forward w stored as 2^32 - w); inverse selects psi^-1.
+ * bytes must equal the table size.  Inspection for tests and tools; the
+ * reference keeps the same tables (tw_fwd / tw_fwd_shoup, solve.cu:1224-1257)
+ * on its device with no such accessor. */
+int nc_plan_copy_twiddles(const nc_plan* plan, uint32_t width, int inverse, void* out, size_t bytes);
+/* The same table built entirely on the host (no device), in the same
+ * layout: the reference's host construction (solve.cu:1224-1257) in this
+ * library's merged-psi form.  *bytes: capacity of out on entry, the table
+ * size on return; out == NULL only reports the size. */
+int nc_host_twiddles(uint32_t N, uint32_t L, const uint32_t* q, uint32_t width, int inverse, void* out,
+                     size_t* bytes);
 /* Pin every call to prime width `width` (62 or 30) and NTT outer split
  * `log_blocks` (0xffffffff: a default for that width).  width 0 returns to
  * the autotuned / default choices.  Any accepted setting yields bit-identical
