@@ -278,6 +278,16 @@ def main() -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    # plan set-up (the fherma_init equivalent, untimed by the reference):
+    # host constants and self tests, device twiddle tables, upload
+    create_ms = {}
+    for n_create in (N, 1 << 16):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ncmul.Plan(n_create, q).close()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        create_ms[str(n_create)] = min(ts)
     batched = ncmul.Plan(N, q)
     plain = ncmul.Plan(N, q)
     if args.tuning:
@@ -393,6 +403,7 @@ def main() -> int:
             "p50_ms": p50, "p99_ms": p99, "calls": calls,
         },
         "batched_over_plain": batched_pps / plain_dev_pps,
+        "plan_create_ms": create_ms,
         "p99_ms": p99,
         "bit_exact": bit_exact,
         "stage_ms": stage_ms,
