@@ -386,7 +386,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64",
+        # residue words of the prime set the batched path runs (30-bit primes:
+        # u32 words with 64-bit products; 62-bit primes: u64 words)
+        "dtype": "u32" if w_batch == 30 else "u64",
         "data": "synthetic (SplitMix64 seeds, uniform in [0, q))",
         "config": {
             "workload": f"negacyclic product mod (X^{N}+1, q), q = 2^868-467 (L=28 u32 limbs), "
