@@ -321,13 +321,16 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
     }
 }
 
-template <class A, int R, int NE>
+// MAY_TOP: the pass can hold global stage 0 (bit log_n - 1), which can only
+// be its last stage kk = R - 1; passes over lower bits never do, and skip
+// the run-time test (and the second copy of every stage it would compile).
+template <class A, int R, int NE, bool MAY_TOP = true>
 __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, int lo, int log_n,
                                         const typename A::Twid* __restrict__ tw, const typename A::Prime& c) {
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
         const int beta = lo + kk;
-        if (beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
+        if (MAY_TOP && kk == R - 1 && beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
                 const int ep = e & ((1 << R) - 1);
@@ -422,7 +425,7 @@ template <class A, int LOGB, int M>
 __device__ __forceinline__ void inverse_from(typename A::T (&v)[16], typename A::T* lds, uint32_t tid, uint32_t gblk,
                                              int log_n, const typename A::Twid* tw, const typename A::Prime& c) {
     using P = PassMap<LOGB, M>;
-    gs_pass<A, P::R, 16>(v, gblk | P::base(tid), P::LO, log_n, tw, c);
+    gs_pass<A, P::R, 16, M == 0>(v, gblk | P::base(tid), P::LO, log_n, tw, c);
     if constexpr (M > 0) {
         exchange<A, LOGB, M, M - 1>(v, lds, tid);
         inverse_from<A, LOGB, M - 1>(v, lds, tid, gblk, log_n, tw, c);
