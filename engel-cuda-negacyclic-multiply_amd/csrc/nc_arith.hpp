@@ -47,10 +47,18 @@ struct W30 {
     // x - m if x >= m else x: when x < m, x - m wraps above x (m < 2^31), so
     // the minimum picks the right one in two instructions.
     __device__ __forceinline__ static T sub_if_ge(T x, T m) { return min(x, x - m); }
+    // a b 2^-32 mod p in (0, 2p) for a, b < 2p: with m = lo p^-1 mod 2^32
+    // (p^-1 = -mneg), m p has the low word of a b, so
+    // (a b - m p) / 2^32 = hi - hi(m p), exactly, in (-p, p) as 4p^2 < p 2^32.
+    // The + p rides in the product's multiply-add (a b + p 2^32 < 2^64).
     __device__ __forceinline__ static T mont(T a, T b, T p, T mneg) {
-        const uint64_t t = (uint64_t)a * b;
-        const uint32_t lo = (uint32_t)t, hi = (uint32_t)(t >> 32);
-        return hi + __umulhi(lo * mneg, p) + (lo != 0);
+        uint64_t p_hi = (uint64_t)p << 32;
+        asm("" : "+v"(p_hi));  // a register-pair addend of the multiply-add
+        const uint64_t t = (uint64_t)a * b + p_hi;
+        const uint32_t lo = (uint32_t)t, hi_p = (uint32_t)(t >> 32);
+        T pinv = 0u - mneg;
+        asm("" : "+v"(pinv));  // keep the inverse: no negate per product
+        return hi_p - __umulhi(lo * pinv, p);
     }
 };
 
