@@ -63,6 +63,11 @@ uint32_t mfma_available(const nc_plan* p) {
 }
 
 // Paths a call with choice c actually runs.
+// The last two inverse NTT stages run inside the CRT (split 2, matrix-core CRT).
+bool inv_in_crt(const nc_plan* p, const Choice& c) {
+    return nc::crt_fuses_outer_inv(p->dev, (int)c.width, (c.mfma & kMfmaCrt) != 0, c.log_blocks);
+}
+
 uint32_t mfma_in_use(const nc_plan* p, const Choice& c) { return c.width == 30 ? (c.mfma & mfma_available(p)) : 0u; }
 
 Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
@@ -138,9 +143,14 @@ int run_chunked(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* 
                                     dm, fuse ? c.log_blocks : 0u),
                "decompose launch");
         if (i == 0) NC_HIP(hipEventRecord(p->ev_first, st), "stagger");
-        NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, wa, wb, cnt, c.log_blocks, st, fuse), "ntt launch");
-        NC_HIP(nc::launch_crt(p->dev, (int)c.width, wa, d_c + off * words, cnt, st, (c.mfma & kMfmaCrt) != 0),
-               "crt launch");
+        const bool inv = inv_in_crt(p, c);
+        NC_HIP(nc::launch_ntt_product(p->dev, (int)c.width, wa, wb, cnt, c.log_blocks, st, fuse, inv), "ntt launch");
+        if (inv)
+            NC_HIP(nc::launch_crt_inv2(p->dev, static_cast<const uint32_t*>(wa), d_c + off * words, cnt, st),
+                   "crt launch");
+        else
+            NC_HIP(nc::launch_crt(p->dev, (int)c.width, wa, d_c + off * words, cnt, st, (c.mfma & kMfmaCrt) != 0),
+                   "crt launch");
     }
     for (int k = 0; k < 2; ++k) {
         NC_HIP(hipEventRecord(p->ev_join[k], p->sub[k]), "join");
@@ -177,8 +187,12 @@ int run_pipeline_ws(nc_plan* p, void* ws_a, void* ws_b, const uint32_t* d_a, con
     const bool fuse = decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
     NC_HIP(launch_decompose(p->dev, (int)c.width, d_a, ws_a, batch, d_b, ws_b, batch, s, dm, fuse ? c.log_blocks : 0u),
            "decompose launch");
-    NC_HIP(launch_ntt_product(p->dev, (int)c.width, ws_a, ws_b, batch, c.log_blocks, s, fuse), "ntt launch");
-    NC_HIP(launch_crt(p->dev, (int)c.width, ws_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
+    const bool inv = inv_in_crt(p, c);
+    NC_HIP(launch_ntt_product(p->dev, (int)c.width, ws_a, ws_b, batch, c.log_blocks, s, fuse, inv), "ntt launch");
+    if (inv)
+        NC_HIP(launch_crt_inv2(p->dev, static_cast<const uint32_t*>(ws_a), d_c, batch, s), "crt launch");
+    else
+        NC_HIP(launch_crt(p->dev, (int)c.width, ws_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0), "crt launch");
     return NC_OK;
 }
 
@@ -749,9 +763,12 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
         if (e == hipSuccess) e = hipEventRecord(ev[2], s);
         if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-        if (e == hipSuccess) e = nc::launch_ntt_outer_inv(p->dev, w, p->d_res_a, batch, c.log_blocks, s);
+        const bool inv = inv_in_crt(p, c);  // then the CRT time includes the last inverse stages
+        if (e == hipSuccess && !inv) e = nc::launch_ntt_outer_inv(p->dev, w, p->d_res_a, batch, c.log_blocks, s);
         if (e == hipSuccess) e = hipEventRecord(ev[4], s);
-        if (e == hipSuccess) e = nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0);
+        if (e == hipSuccess)
+            e = inv ? nc::launch_crt_inv2(p->dev, static_cast<const uint32_t*>(p->d_res_a), d_c, batch, s)
+                    : nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0);
         if (e == hipSuccess) e = hipEventRecord(ev[5], s);
         if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
         for (int i = 0; i < 5 && e == hipSuccess; ++i) {

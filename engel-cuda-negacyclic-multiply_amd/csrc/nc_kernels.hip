@@ -1037,12 +1037,27 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 // KCT); KCT == 0 takes kc_n <= kCrtMaxKC at run time.  LT > 0 fixes L.
 // A wave's 64 results (64 L contiguous words of out) leave through its own
 // LDS region as 16-byte stores.
-template <int RT, int KCT, int LT>
+//
+// INV == 2 also runs the last two inverse NTT stages (what
+// ntt_outer_inv_kernel<W30, 2> does between the inner kernel and the CRT):
+// a wave then takes positions x0 + i (i < 16) with their partners N/4, N/2
+// and 3N/4 away -- tile 0 column c holds x0 + (c & 15) + (c >> 4) N/2,
+// tile 1 the same + N/4 -- so the stage of span N/4 pairs a lane's own two
+// tiles and the stage of span N/2 (N^-1 R y_j merged) pairs lanes c and
+// c ^ 16; the residues become canonical before the contraction, and the
+// results leave as four runs of 16 coefficients.
+struct CrtInvLds {
+    uint32_t p, nr, nr_s, wr, wr_s, w2, w2_s, w3, w3_s;
+};
+
+template <int RT, int KCT, int LT, int INV = 0>
 __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res, uint32_t* __restrict__ out,
                                                 const int8_t* __restrict__ tab, const int64_t* __restrict__ corr,
                                                 const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
                                                 uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n,
-                                                uint32_t nprod) {
+                                                uint32_t nprod, const PrimeConst32* __restrict__ pcs = nullptr,
+                                                const Tw32* __restrict__ twi = nullptr) {
+    static_assert(INV == 0 || INV == 2, "fused inverse stages: 0 or 2");
     constexpr int LF = 8 * RT - 4;                         // fraction pseudo-limbs LF, LF + 1
     constexpr int LQ = LF < kMaxLimbs ? LF : kMaxLimbs;  // q limbs (zero beyond L)
     constexpr int KCM = KCT > 0 ? KCT : kCrtMaxKC;
@@ -1057,7 +1072,19 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
         *reinterpret_cast<nc_v4i*>(lds_tab + row * kCrtRowStride + ch * 16) =
             *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kbytes + ch * 16);
     }
+    // entries 8 kc + 4 h + e for every chunk; rows >= k repeat row k - 1
+    __shared__ CrtInvLds lds_inv[INV ? 8 * KCM : 1];
+    if constexpr (INV > 0) {
+        for (uint32_t r = threadIdx.x; r < 8u * KCM; r += 256) {
+            const uint32_t j = r < k ? r : k - 1;
+            const PrimeConst32 c = pcs[j];
+            const Tw32 t2 = twi[((size_t)j << log_n) + 2], t3 = twi[((size_t)j << log_n) + 3];
+            lds_inv[r] = CrtInvLds{c.p, c.nr, c.nr_s, c.wr, c.wr_s, t2.w, t2.ws, t3.w, t3.ws};
+        }
+    }
     __syncthreads();
+    // kc_n chunks of 8 primes: k > 8 (KCT - 1), so only the last chunk has rows >= k
+    if (KCT > 0 && k <= 8u * (KCT - 1)) __builtin_unreachable();
     const uint32_t N = 1u << log_n;
     const uint32_t total = nprod << log_n;  // a multiple of 32 (N >= 32)
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -1068,14 +1095,19 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
     // every load issued before the first use; rows j >= k read row k - 1 and
     // are replaced by 0 at use
     nc_v4i bf[2][KCM];
+    // INV: the wave's first position and the position of this lane's column of a tile
+    auto inv_x0 = [&](uint32_t g0) { return ((g0 & (N - 1)) >> 6) * 16u; };
+    auto inv_pos = [&](uint32_t x0, int tile, uint32_t c) {
+        return x0 + (c & 15u) + (c >> 4) * (N >> 1) + (uint32_t)tile * (N >> 2);
+    };
     auto load_group = [&](uint32_t g0) {
 #pragma unroll
         for (int tile = 0; tile < 2; ++tile) {
             const uint32_t t0 = g0 + 32u * tile;
-            const uint32_t tv = t0 < total ? t0 : g0;  // a missing second tile re-reads the first
-            const uint32_t prod = uniform(tv >> log_n);
+            const uint32_t tv = (INV || t0 < total) ? t0 : g0;  // a missing second tile re-reads the first
+            const uint32_t prod = uniform((INV ? g0 : tv) >> log_n);
             const uint32_t* base = res + (size_t)prod * k * N;
-            const uint32_t off0 = (tv & (N - 1)) + col;
+            const uint32_t off0 = INV ? inv_pos(inv_x0(g0), tile, col) : (tv & (N - 1)) + col;
 #pragma unroll
             for (int kc = 0; kc < KCM; ++kc) {
                 if ((uint32_t)kc < kc_n) {  // wave-uniform
@@ -1093,6 +1125,35 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
         const uint32_t g0 = grp * 256u + wave * 64u;
         if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
         load_group(g0);
+        if constexpr (INV > 0) {
+            // last two inverse stages, then canonical residues (rows >= k are
+            // replaced by 0 at use; their constants are row k - 1's).  The
+            // memory clobber keeps the per-prime constants from being hoisted
+            // out of the group loop (32 primes x 9 words would not fit).
+            asm volatile("" ::: "memory");
+            const bool lower = col < 16u;
+#pragma unroll
+            for (int kc = 0; kc < KCM; ++kc) {
+                if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const CrtInvLds ci = lds_inv[8 * kc + 4 * half + e];
+                    const uint32_t p = ci.p, p2 = 2 * p;
+                    // span N/4: tile 0 (x) with tile 1 (x + N/4); twiddle psi^-bitrev(2 or 3)
+                    const Tw32 w = lower ? Tw32{ci.w2, ci.w2_s} : Tw32{ci.w3, ci.w3_s};
+                    uint32_t a = (uint32_t)bf[0][kc][e], b = (uint32_t)bf[1][kc][e];
+                    Bfly<W30>::gs(a, b, w, p, p2);
+                    // span N/2 (stage 0): lane c pairs with lane c ^ 16
+                    const uint32_t pa = (uint32_t)__shfl_xor((int)a, 16);
+                    const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
+                    const uint32_t sa = lower ? a + pa : pa - a + p2;
+                    const uint32_t sb = lower ? b + pb : pb - b + p2;
+                    const uint32_t wf = lower ? ci.nr : ci.wr, wfs = lower ? ci.nr_s : ci.wr_s;
+                    bf[0][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sa, wf, wfs, p), p);
+                    bf[1][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sb, wf, wfs, p), p);
+                }
+            }
+        }
         // One row tile: both tiles' contractions (the A fragment serves both),
         // then the exchange.  Lane (c, 0) keeps tile 0's even pseudo-limbs and
         // receives its odd ones; lane (c, 1) receives tile 1's even ones and
@@ -1185,10 +1246,22 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             if ((uint32_t)l < L) st[lane * L + l] = r[l];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t words = (total - g0 < 64u ? total - g0 : 64u) * L;  // a multiple of 4 (32 divides the count)
-        uint32_t* dst = out + (size_t)g0 * L;
-        for (uint32_t w = lane * 4u; w < words; w += 256u)
-            *reinterpret_cast<nc_v4i*>(dst + w) = *reinterpret_cast<const nc_v4i*>(st + w);
+        if constexpr (INV > 0) {
+            // four runs of 16 coefficients: lanes 16 r .. 16 r + 15 hold run r
+            const uint32_t x0 = inv_x0(g0), run_words = 16u * L;  // a multiple of 4
+            uint32_t* dst = out + (size_t)(g0 >> log_n) * N * L;
+            for (uint32_t w = lane * 4u; w < 64u * L; w += 256u) {
+                const uint32_t r = w / run_words, within = w - r * run_words;
+                const uint32_t start = x0 + (r & 1u) * (N >> 1) + (r >> 1) * (N >> 2);
+                *reinterpret_cast<nc_v4i*>(dst + (size_t)start * L + within) =
+                    *reinterpret_cast<const nc_v4i*>(st + w);
+            }
+        } else {
+            const uint32_t words = (total - g0 < 64u ? total - g0 : 64u) * L;  // a multiple of 4 (32 divides the count)
+            uint32_t* dst = out + (size_t)g0 * L;
+            for (uint32_t w = lane * 4u; w < words; w += 256u)
+                *reinterpret_cast<nc_v4i*>(dst + w) = *reinterpret_cast<const nc_v4i*>(st + w);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
@@ -1196,12 +1269,13 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
 
 // The fixed-shape instantiation fits two waves per SIMD without spilling;
 // the run-time-shaped one is left to the compiler's register budget.
-template <int RT, int KCT, int LT>
+template <int RT, int KCT, int LT, int INV = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void crt30_mfma_kernel(
     const uint32_t* __restrict__ res, uint32_t* __restrict__ out, const int8_t* __restrict__ tab,
     const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
-    uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod) {
-    crt30_mfma_body<RT, KCT, LT>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod);
+    uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod, const PrimeConst32* __restrict__ pcs,
+    const Tw32* __restrict__ twi) {
+    crt30_mfma_body<RT, KCT, LT, INV>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod, pcs, twi);
 }
 
 template <int RT>
@@ -1501,13 +1575,13 @@ hipError_t launch_ntt_outer_inv(const DevPlan& d, int width, void* ra, uint32_t 
 }
 
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
-                              hipStream_t s, bool outer_fwd_done) {
+                              hipStream_t s, bool outer_fwd_done, bool outer_inv_in_crt) {
     if (d.log_n >= 4 && (log_blocks < min_log_blocks(d.log_n) || log_blocks > max_log_blocks(d.log_n)))
         return hipErrorInvalidValue;
     hipError_t e = outer_fwd_done ? hipSuccess : launch_ntt_outer_fwd(d, width, ra, rb, nprod, log_blocks, s);
     if (e != hipSuccess) return e;
     e = launch_ntt_inner(d, width, ra, rb, nprod, log_blocks, s);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || outer_inv_in_crt) return e;
     return launch_ntt_outer_inv(d, width, ra, nprod, log_blocks, s);
 }
 
@@ -1522,10 +1596,29 @@ static void launch_crt_mfma(const DevPlan& d, const uint32_t* res, uint32_t* out
     const uint32_t grid = groups < cap ? groups : cap;
     if constexpr (KCT > 0)
         crt30_mfma_kernel<RT, KCT, LT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
-                                                         d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod);
+                                                         d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, nullptr,
+                                                         nullptr);
     else
         crt30_mfma_generic_kernel<RT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc,
                                                             d.w30.qc, d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod);
+}
+
+bool crt_fuses_outer_inv(const DevPlan& d, int width, bool mfma, uint32_t log_blocks) {
+    return width == 30 && mfma && crt_mfma_available(d) && log_blocks == 2 && d.log_n >= 6 && d.w30.crt_rt == 4 &&
+           d.w30.crt_kc == 8 && d.L == 28;
+}
+
+hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out, uint32_t nprod, hipStream_t s) {
+    if (!crt_fuses_outer_inv(d, 30, true, 2)) return hipErrorInvalidValue;
+    constexpr uint32_t lds = 4u * 32u * kCrtRowStride + 4u * 64u * 4u * 28u;
+    const uint32_t groups = ((nprod << d.log_n) + 255) / 256;
+    const uint32_t per_cu = (64u * 1024u * 5u / 2u) / (lds + (uint32_t)(kMaxPrimes30 * sizeof(CrtInvLds)));
+    const uint32_t cap = d.cus * (per_cu < 2u ? per_cu : 2u);
+    const uint32_t grid = groups < cap ? groups : cap;
+    crt30_mfma_kernel<4, 8, 28, 2><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
+                                                        d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, d.w30.pc,
+                                                        d.w30.tw_inv);
+    return hipGetLastError();
 }
 
 hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* out, uint32_t nprod, hipStream_t s,

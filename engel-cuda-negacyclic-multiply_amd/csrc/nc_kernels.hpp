@@ -70,7 +70,7 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
 // [0, p)); rb is clobbered.  log_blocks is the outer
 // split (0 = the whole row in one workgroup).
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
-                              hipStream_t s, bool outer_fwd_done = false);
+                              hipStream_t s, bool outer_fwd_done = false, bool outer_inv_in_crt = false);
 
 // The three parts of launch_ntt_product, for per-stage timing.
 hipError_t launch_ntt_outer_fwd(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
@@ -93,6 +93,13 @@ hipError_t launch_twiddles62(Tw* fwd, Tw* inv, const Tw* pow, const PrimeConst* 
                              hipStream_t s);
 hipError_t launch_twiddles30(Tw32* fwd, Tw32* inv, const Tw32* pow, const PrimeConst32* pc, uint32_t log_n,
                              uint32_t k, hipStream_t s);
+
+// The 30-bit matrix-core CRT that also runs the last two inverse NTT stages
+// (outer split 2): res holds the inner kernel's output (lazy [0, 2p]), and
+// launch_ntt_product's outer inverse is skipped.  Available for the fixed
+// shape L = 28 with 57..64 primes.
+bool crt_fuses_outer_inv(const DevPlan& d, int width, bool mfma, uint32_t log_blocks);
+hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out, uint32_t nprod, hipStream_t s);
 
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
