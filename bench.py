@@ -357,6 +357,10 @@ def main() -> int:
     if fused:  # the outer forward stages ran inside the decomposition kernel
         work["decompose"] += work["ntt_outer_fwd"]
         work["ntt_outer_fwd"] = byts["ntt_outer_fwd"] = 0
+    inv_fused = s1_batch > 0 and stage_ms["ntt_outer_inv"] == 0.0
+    if inv_fused:  # the last inverse stages ran inside the CRT kernel
+        work["crt"] += work["ntt_outer_inv"]
+        work["ntt_outer_inv"] = byts["ntt_outer_inv"] = 0
     peak_mul = ncmul.measure_mul32_peak()
     dom = max(stage_ms, key=lambda s: stage_ms[s])
     dom_s = stage_ms[dom] * 1e-3
@@ -403,7 +407,8 @@ def main() -> int:
             "N": N, "L": L, "batch": B, "k62": batched.k62, "k30": batched.k30,
             "batched": {"prime_width": w_batch, "ntt_split": s1_batch, "mfma_paths": batched.choice(B)[2],
                         "chunks": batched.choice(B)[3],
-                        "outer_forward_fused_into_decompose": fused},
+                        "outer_forward_fused_into_decompose": fused,
+                        "outer_inverse_fused_into_crt": inv_fused},
             "plain": {"prime_width": w_plain, "ntt_split": s1_plain, "mfma_paths": plain.choice(1)[2]},
         },
         "plain": {
