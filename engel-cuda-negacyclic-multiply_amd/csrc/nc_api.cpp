@@ -784,6 +784,7 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     for (int i = 0; i < 5; ++i) ms[i] = (float)(acc[i] / reps);
     if (!outer) ms[1] = ms[3] = 0.f;
     if (fuse) ms[1] = 0.f;  // the outer forward stages ran inside the decomposition
+    if (inv_in_crt(p, c)) ms[3] = 0.f;  // the last inverse stages ran inside the CRT
     return NC_OK;
 }
 

@@ -178,7 +178,10 @@ int nc_stage_crt(nc_plan* plan, uint32_t width, const void* d_res, uint32_t* d_c
  * with HIP events recorded on that stream between the launches:
  *   ms[0] decompose, ms[1] NTT outer forward, ms[2] NTT inner (forward,
  *   pointwise, inverse), ms[3] NTT outer inverse, ms[4] CRT.
- * A stage the plan's split does not launch reports 0.  Blocks until done.
+ * A stage the plan's split does not launch reports 0, and so does a stage
+ * that runs inside a neighbour (the outer forward stages inside the
+ * decomposition, the outer inverse stages inside the CRT): its time is in
+ * that neighbour's figure.  Blocks until done.
  * (No counterpart in the reference, which has no timing or tracing.) */
 int nc_profile_stages(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
                       uint32_t reps, void* stream, float* ms);
