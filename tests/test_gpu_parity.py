@@ -169,9 +169,11 @@ def test_stage_product_residues_and_crt_30(gpu, q868, mfma):
 
 
 @pytest.mark.parametrize("width", WIDTHS)
-@pytest.mark.parametrize("N", [16, 1024, 16384, 65536])
+@pytest.mark.parametrize("N", [16, 64, 256, 1024, 16384, 65536])
 def test_tuning_invariance(gpu, N, width, q868):
-    """Every accepted NTT split gives the same bits as the oracle."""
+    """Every accepted NTT split gives the same bits as the oracle (split 2
+    with the matrix-core CRT runs the last inverse stages inside the CRT;
+    N = 64 is its smallest case, one wave per product)."""
     a, b = ncmul.gen_uniform(41, N, q868), ncmul.gen_uniform(42, N, q868)
     want = O.Plan(N, q868).multiply(a, b)
     with ncmul.Plan(N, q868) as plan:
@@ -218,7 +220,8 @@ def test_chunked_batch(gpu, width, chunks, q868):
     b = np.stack([ncmul.gen_uniform(301 + 2 * i, N, q868) for i in range(B)])
     orc = O.Plan(N, q868)
     with ncmul.Plan(N, q868) as plan:
-        plan.set_tuning(width)
+        # 30-bit: split 2, so chunks also run the CRT with the inverse stages inside
+        plan.set_tuning(width, 2 if width == 30 else None)
         plan.set_chunks(chunks)
         assert plan.choice(B)[3] == min(chunks, B)
         d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
