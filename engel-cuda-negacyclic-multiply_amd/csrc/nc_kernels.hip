@@ -26,6 +26,9 @@
 // (sub-)workgroup, 16 values per thread in registers, LDS for the exchanges
 // between radix-16 passes.  The inner kernel also does the pointwise product,
 // so forward-inner, pointwise and inverse-inner share one HBM round trip.
+// The outer stages can ride in neighbours instead of their own pass: the
+// matrix-core decomposition runs s1 <= 2 forward stages, and the
+// matrix-core CRT runs the last two inverse stages at s1 = 2.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

@@ -68,7 +68,10 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
 // product, v_j = (c mod p_j) (M / p_j)^-1 mod p_j (the inverse transform's
 // merged N^-1 constant carries the CRT weight), in place in ra (canonical
 // [0, p)); rb is clobbered.  log_blocks is the outer
-// split (0 = the whole row in one workgroup).
+// split (0 = the whole row in one workgroup).  outer_fwd_done: the
+// decomposition already ran the outer forward stages.  outer_inv_in_crt:
+// stop after the inner kernel (lazy [0, 2p] values) and let
+// launch_crt_inv2 run the outer inverse stages.
 hipError_t launch_ntt_product(const DevPlan& d, int width, void* ra, void* rb, uint32_t nprod, uint32_t log_blocks,
                               hipStream_t s, bool outer_fwd_done = false, bool outer_inv_in_crt = false);
 
