@@ -385,7 +385,7 @@ struct PassMap {
     using S = InnerShape<LOGB>;
     static constexpr int LO = LOGB - S::R0 - 4 * M;
     static constexpr int R = M == 0 ? S::R0 : 4;
-    __device__ __forceinline__ static uint32_t base(uint32_t tid) {
+    static constexpr uint32_t base(uint32_t tid) {
         if constexpr (M == 0) return tid << (4 - S::R0);
         else return (tid & ((1u << LO) - 1)) | ((tid >> LO) << (LO + 4));
     }
@@ -396,7 +396,43 @@ struct PassMap {
             return (uint32_t)e << LO;
         }
     }
+    // index bits that one wave's lanes and registers span (base and delta
+    // are bit permutations of tid and e)
+    static constexpr uint32_t wave_bits() {
+        uint32_t m = 0;
+        for (int i = 0; i < 6; ++i) m |= base(1u << i);
+        for (int i = 0; i < 4; ++i) m |= delta(1 << i);
+        return m;
+    }
 };
+
+// An exchange between two pass layouts is wave-local when every wave holds
+// the same set of indices before and after it: the whole block lives in one
+// wave, or both layouts give each wave the same spanned bits and the same
+// fixed bits.  Such an exchange needs no workgroup barrier: the region a wave
+// writes is the one it alone read in the exchange before (whose target
+// layout is this one's source), and LDS serves one wave's accesses in order.
+template <int LOGB, int FROM, int TO>
+constexpr bool exchange_in_wave() {
+    using S = InnerShape<LOGB>;
+    using F = PassMap<LOGB, FROM>;
+    using G = PassMap<LOGB, TO>;
+    if (S::TPB <= 64) return true;
+    if (F::wave_bits() != G::wave_bits()) return false;
+    for (uint32_t w = 0; w < (uint32_t)S::TPB / 64; ++w)
+        if (F::base(64 * w) != G::base(64 * w)) return false;
+    return true;
+}
+
+template <bool IN_WAVE>
+__device__ __forceinline__ void exchange_sync() {
+    if constexpr (IN_WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
 
 __host__ __device__ constexpr uint32_t lds_pad(uint32_t x) { return x + (x >> 4); }
 
@@ -404,12 +440,13 @@ template <class A, int LOGB, int FROM, int TO>
 __device__ __forceinline__ void exchange(typename A::T (&v)[16], typename A::T* lds, uint32_t tid) {
     using F = PassMap<LOGB, FROM>;
     using G = PassMap<LOGB, TO>;
+    constexpr bool kWave = exchange_in_wave<LOGB, FROM, TO>();
     typename A::T* w = lds + lds_pad(F::base(tid));
     typename A::T* r = lds + lds_pad(G::base(tid));
-    __syncthreads();
+    exchange_sync<kWave>();
 #pragma unroll
     for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = v[e];
-    __syncthreads();
+    exchange_sync<kWave>();
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = r[lds_pad(G::delta(e))];
 }
@@ -575,12 +612,13 @@ template <int LOGB, int FROM, int TO>
 __device__ __forceinline__ void exchange_pair(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid) {
     using F = PassMap<LOGB, FROM>;
     using G = PassMap<LOGB, TO>;
+    constexpr bool kWave = exchange_in_wave<LOGB, FROM, TO>();
     uint2* w = lds + lds_pad(F::base(tid));
     uint2* r = lds + lds_pad(G::base(tid));
-    __syncthreads();
+    exchange_sync<kWave>();
 #pragma unroll
     for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = make_uint2(va[e], vb[e]);
-    __syncthreads();
+    exchange_sync<kWave>();
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const uint2 x = r[lds_pad(G::delta(e))];
