@@ -233,10 +233,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             const char* cp = reinterpret_cast<const char*>(src + (size_t)(c0 + e * span) * L);
 #pragma unroll
             for (int ks = 0; ks < kMfmaK / 32; ++ks) {
+                // bytes m >= 4L meet zero table entries, so any bytes serve
+                // there: read the coefficient's last 16 instead.  The load
+                // stays unconditional, and all of them issue before one wait
+                // (a guarded load waits for itself inside its branch).
                 const uint32_t off = 32 * ks + 16 * half;
-                nc_v4i x = {0, 0, 0, 0};
-                if (off < nbytes) x = *reinterpret_cast<const nc_v4i*>(cp + off);
-                bf[e][ks] = x ^ (int)0x80808080;  // bytes a_m - 128; padding bytes meet zero table entries
+                const uint32_t offc = off < nbytes ? off : nbytes - 16;
+                bf[e][ks] = *reinterpret_cast<const nc_v4i*>(cp + offc) ^ (int)0x80808080;  // bytes a_m - 128
             }
         }
         for (uint32_t t = 0; t < tiles; ++t) {
