@@ -1150,16 +1150,29 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             const uint32_t t0 = g0 + 32u * tile;
             const uint32_t tv = (INV || t0 < total) ? t0 : g0;  // a missing second tile re-reads the first
             const uint32_t prod = uniform((INV ? g0 : tv) >> log_n);
-            const uint32_t* base = res + (size_t)prod * k * N;
+            const char* base = reinterpret_cast<const char*>(res + (size_t)prod * k * N);
             const uint32_t off0 = INV ? inv_pos(inv_x0(g0), tile, col) : (tv & (N - 1)) + col;
+            // lane half h reads row 8 kc + 4 h + e: the row 8 kc + e is
+            // wave-uniform (a scalar base), the lane's byte offset in it is
+            // 4 off0, or 4 (off0 + 4 N) for the upper half.  Where row
+            // 8 kc + 4 h + e >= k the lane reads row min(8 kc + e, k - 1)
+            // instead (in bounds; such values are replaced by 0 at use).
+            const uint32_t lo_b = 4u * off0, hi_b = 4u * (off0 + 4u * N);
+            // the row pitch in a register the compiler cannot see through:
+            // the 64 row offsets are then recomputed (scalar) at each use
+            // rather than hoisted out of the group loop, which they would
+            // overflow the scalar registers in
+            uint32_t pitch = 4u * N;
+            asm volatile("" : "+s"(pitch));
 #pragma unroll
             for (int kc = 0; kc < KCM; ++kc) {
                 if ((uint32_t)kc < kc_n) {  // wave-uniform
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const uint32_t j = 8 * kc + 4 * half + e;
-                        const uint32_t row = j < k ? j : k - 1;
-                        bf[tile][kc][e] = (int)base[row * N + off0];
+                        const uint32_t ra = 8 * kc + e;
+                        const uint32_t rowu = ra < k ? ra : k - 1;
+                        const uint32_t lb = (half && ra + 4 < k) ? hi_b : lo_b;
+                        bf[tile][kc][e] = *reinterpret_cast<const int*>(base + rowu * pitch + lb);
                     }
                 }
             }
