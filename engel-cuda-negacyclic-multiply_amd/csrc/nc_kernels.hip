@@ -502,7 +502,10 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
     uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * S::BPW + sub;
     if constexpr (S::TPB >= 64) wid = uniform(wid);  // whole waves per block: scalar prime constants
     const bool active = wid < total;
-    if (!active) wid = 0;  // idle sub-block: keeps barriers uniform, touches nothing
+    // an idle sub-block works on block 0 (its loads stay in bounds and
+    // unconditional, so none of them needs a branch) for uniform barriers,
+    // and stores nothing
+    if (!active) wid = 0;
     const uint32_t blk = wid & ((1u << nblk_log) - 1);
     const uint32_t row = wid >> nblk_log;  // = j * nprod + prod
     const uint32_t j = row / nprod, prod = row - j * nprod;
@@ -520,7 +523,7 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
     const T* gb0 = rb + off + P0::base(tid);
     T v[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = active ? ga0[P0::delta(e)] : T(0);
+    for (int e = 0; e < 16; ++e) v[e] = ga0[P0::delta(e)];
     forward_from<A, LOGB, 0>(v, lds, tid, gblk, (int)log_n, tf, c.p, c.p2);
     if (active) {
 #pragma unroll
@@ -528,11 +531,11 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = active ? gb0[P0::delta(e)] : T(0);
+    for (int e = 0; e < 16; ++e) v[e] = gb0[P0::delta(e)];
     forward_from<A, LOGB, 0>(v, lds, tid, gblk, (int)log_n, tf, c.p, c.p2);
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const T x = active ? gal[PL::delta(e)] : T(0);
+        const T x = gal[PL::delta(e)];
         v[e] = A::mont(x, A::sub_if_ge(v[e], c.p2), c.p, c.mont_neg);
     }
     inverse_from<A, LOGB, LAST>(v, lds, tid, gblk, (int)log_n, ti, c);
@@ -657,7 +660,10 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * S::BPW + sub;
     if constexpr (S::TPB >= 64) wid = uniform(wid);
     const bool active = wid < total;
-    if (!active) wid = 0;  // idle sub-block: keeps barriers uniform, touches nothing
+    // an idle sub-block works on block 0 (its loads stay in bounds and
+    // unconditional, so none of them needs a branch) for uniform barriers,
+    // and stores nothing
+    if (!active) wid = 0;
     const uint32_t blk = wid & ((1u << nblk_log) - 1);
     const uint32_t row = wid >> nblk_log;  // = j * nprod + prod
     const uint32_t j = row / nprod, prod = row - j * nprod;
@@ -675,8 +681,8 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     T va[16], vb[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        va[e] = active ? ga0[IO::delta(e)] : T(0);
-        vb[e] = active ? gb0[IO::delta(e)] : T(0);
+        va[e] = ga0[IO::delta(e)];
+        vb[e] = gb0[IO::delta(e)];
     }
     if constexpr (kLinear) exchange_pair_maps<LinearMap<LOGB>, P0>(va, vb, lds2, tid);
     forward_pair_from<LOGB, 0>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2);
