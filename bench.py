@@ -223,8 +223,15 @@ def cpu_baseline(N: int, q: np.ndarray, seconds: float):
 
     done, el = timed(cores, seconds)
     done1, el1 = timed(1, seconds / 2)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {
         "value": done / el, "unit": "products/s", "cores": cores, "kind": "port",
+        "host": {"cpu_model": model, "nproc": os.cpu_count(), "threads_allowed": cores},
         "sample": f"{done} products at N={N}, L=28 (868-bit q) in {el:.1f} s: oracle/nc_oracle.c "
                   f"restatement of solve.cu (prime rule, psi-twisted DIT NTT, Montgomery/Shoup, fast CRT), "
                   f"OpenMP over primes then coefficients",
