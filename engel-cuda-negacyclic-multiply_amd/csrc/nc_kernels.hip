@@ -816,19 +816,26 @@ __global__ __launch_bounds__(64) void ntt_tiny_kernel(typename A::T* __restrict_
 template <int NA>
 __device__ __forceinline__ void mac_limbs(uint32_t (&acc)[NA], const uint32_t* __restrict__ m, int mlen_ct,
                                           uint32_t f, int off) {
-    uint64_t carry = 0;
+    // carry-save: every product f m_i is independent; its low word joins
+    // limb off + i and its high word limb off + i + 1 through two 32-bit
+    // add-with-carry chains (one v_addc_co_u32 each per limb), the carry
+    // out of the top limb is dropped
+    uint32_t ca = 0, cb = 0, hi_prev = 0;
 #pragma unroll
     for (int l = 0; l < NA; ++l) {
         if (l < off) continue;
+        uint32_t lo = 0, hi = 0;
         if (l - off < mlen_ct) {
-            const uint64_t t = (uint64_t)f * m[l - off] + acc[l] + carry;
-            acc[l] = (uint32_t)t;
-            carry = t >> 32;
-        } else {
-            const uint64_t t = (uint64_t)acc[l] + carry;
-            acc[l] = (uint32_t)t;
-            carry = t >> 32;
+            const uint64_t t = (uint64_t)f * m[l - off];
+            lo = (uint32_t)t;
+            hi = (uint32_t)(t >> 32);
         }
+        uint32_t na, nb;
+        const uint32_t a = __builtin_addc(acc[l], lo, ca, &na);
+        acc[l] = __builtin_addc(a, hi_prev, cb, &nb);
+        ca = na;
+        cb = nb;
+        hi_prev = hi;
     }
 }
 
@@ -1045,29 +1052,21 @@ template <int LQ, int NA>
 __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv) {
     static_assert(NA >= LQ + 4, "room for the three steps");
 #pragma unroll
-    for (int st = 0; st < 3; ++st) {
-        const uint32_t u = x[st] * q_inv;
-        uint32_t carry = 0;
-#pragma unroll
-        for (int l = 0; l < NA - st; ++l) {
-            const uint64_t t = (l < LQ ? (uint64_t)u * qc->q[l] : 0ull) + x[st + l] + carry;
-            x[st + l] = (uint32_t)t;
-            carry = (uint32_t)(t >> 32);
-        }
-    }
+    for (int st = 0; st < 3; ++st) mac_limbs<NA>(x, qc->q, LQ, x[st] * q_inv, st);  // x[st] becomes 0
     uint32_t br = 0;  // r = x[3 ..] < 2q: borrow of r - q
 #pragma unroll
     for (int l = 0; l <= LQ; ++l) {
-        const uint64_t t = (uint64_t)x[3 + l] - (l < LQ ? qc->q[l] : 0u) - br;
-        br = (uint32_t)(t >> 63);
+        uint32_t nb;
+        (void)__builtin_subc(x[3 + l], l < LQ ? qc->q[l] : 0u, br, &nb);
+        br = nb;
     }
-    const uint32_t mask = br ? 0u : 0xffffffffu;  // subtract q iff r >= q
+    const bool ge = br == 0;  // subtract q iff r >= q
     br = 0;
 #pragma unroll
     for (int l = 0; l < LQ; ++l) {
-        const uint64_t t = (uint64_t)x[3 + l] - (qc->q[l] & mask) - br;
-        x[3 + l] = (uint32_t)t;
-        br = (uint32_t)(t >> 63);
+        uint32_t nb;
+        x[3 + l] = __builtin_subc(x[3 + l], ge ? qc->q[l] : 0u, br, &nb);
+        br = nb;
     }
 }
 
