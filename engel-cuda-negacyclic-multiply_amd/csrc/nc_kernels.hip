@@ -1095,8 +1095,13 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 // tiles and the stage of span N/2 (N^-1 R y_j merged) pairs lanes c and
 // c ^ 16; the residues become canonical before the contraction, and the
 // results leave as four runs of 16 coefficients.
+// One prime's constants for the fused inverse stages as one half of a
+// column group sees them (entry 2 r + u, u = 1 for lanes c with c & 16):
+// the span-N/4 twiddle psi^-bitrev(2 + u), the stage-0 factor (N^-1 R y_j
+// for the sum, times psi^-bitrev(1) for the difference) and sadd (0 or
+// 2p + 1), so that no lane selects between constants.
 struct CrtInvLds {
-    uint32_t p, nr, nr_s, wr, wr_s, w2, w2_s, w3, w3_s;
+    uint32_t p, w, ws, wf, wfs, sadd, pad0, pad1;
 };
 
 template <int RT, int KCT, int LT, int INV = 0>
@@ -1122,13 +1127,15 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kbytes + ch * 16);
     }
     // entries 8 kc + 4 h + e for every chunk; rows >= k repeat row k - 1
-    __shared__ CrtInvLds lds_inv[INV ? 8 * KCM : 1];
+    __shared__ __attribute__((aligned(16))) CrtInvLds lds_inv[INV ? 2 * 8 * KCM : 1];
     if constexpr (INV > 0) {
-        for (uint32_t r = threadIdx.x; r < 8u * KCM; r += 256) {
+        for (uint32_t r2 = threadIdx.x; r2 < 16u * KCM; r2 += 256) {
+            const uint32_t r = r2 >> 1, u = r2 & 1u;
             const uint32_t j = r < k ? r : k - 1;
             const PrimeConst32 c = pcs[j];
-            const Tw32 t2 = twi[((size_t)j << log_n) + 2], t3 = twi[((size_t)j << log_n) + 3];
-            lds_inv[r] = CrtInvLds{c.p, c.nr, c.nr_s, c.wr, c.wr_s, t2.w, t2.ws, t3.w, t3.ws};
+            const Tw32 t = twi[((size_t)j << log_n) + 2 + u];
+            lds_inv[r2] = u ? CrtInvLds{c.p, t.w, t.ws, c.wr, c.wr_s, 2 * c.p + 1, 0u, 0u}
+                            : CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, 0u, 0u, 0u};
         }
     }
     __syncthreads();
@@ -1193,26 +1200,27 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             // memory clobber keeps the per-prime constants from being hoisted
             // out of the group loop (32 primes x 9 words would not fit).
             asm volatile("" ::: "memory");
-            const bool lower = col < 16u;
+            const uint32_t upper = (col >> 4) & 1u;
+            const uint32_t m = 0u - upper;  // all ones in the upper half
 #pragma unroll
             for (int kc = 0; kc < KCM; ++kc) {
                 if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const CrtInvLds ci = lds_inv[8 * kc + 4 * half + e];
+                    const CrtInvLds ci = lds_inv[2 * (8 * kc + 4 * half + e) + upper];
                     const uint32_t p = ci.p, p2 = 2 * p;
-                    // span N/4: tile 0 (x) with tile 1 (x + N/4); twiddle psi^-bitrev(2 or 3)
-                    const Tw32 w = lower ? Tw32{ci.w2, ci.w2_s} : Tw32{ci.w3, ci.w3_s};
+                    // span N/4: tile 0 (x) with tile 1 (x + N/4)
                     uint32_t a = (uint32_t)bf[0][kc][e], b = (uint32_t)bf[1][kc][e];
-                    Bfly<W30>::gs(a, b, w, p, p2);
-                    // span N/2 (stage 0): lane c pairs with lane c ^ 16
+                    Bfly<W30>::gs(a, b, Tw32{ci.w, ci.ws}, p, p2);
+                    // span N/2 (stage 0): lane c pairs with lane c ^ 16; the
+                    // lower half forms a + pa, the upper pa - a + 2p, both as
+                    // pa + (a ^ m) + sadd
                     const uint32_t pa = (uint32_t)__shfl_xor((int)a, 16);
                     const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
-                    const uint32_t sa = lower ? a + pa : pa - a + p2;
-                    const uint32_t sb = lower ? b + pb : pb - b + p2;
-                    const uint32_t wf = lower ? ci.nr : ci.wr, wfs = lower ? ci.nr_s : ci.wr_s;
-                    bf[0][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sa, wf, wfs, p), p);
-                    bf[1][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sb, wf, wfs, p), p);
+                    const uint32_t sa = pa + (a ^ m) + ci.sadd;
+                    const uint32_t sb = pb + (b ^ m) + ci.sadd;
+                    bf[0][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p), p);
+                    bf[1][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p), p);
                 }
             }
         }
@@ -1674,7 +1682,8 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     if (!crt_fuses_outer_inv(d, 30, true, 2)) return hipErrorInvalidValue;
     constexpr uint32_t lds = 4u * 32u * kCrtRowStride + 4u * 64u * 4u * 28u;
     const uint32_t groups = ((nprod << d.log_n) + 255) / 256;
-    const uint32_t per_cu = (64u * 1024u * 5u / 2u) / (lds + (uint32_t)(kMaxPrimes30 * sizeof(CrtInvLds)));
+    // + the per-lane-half inverse-stage constants of 8 chunks of 8 primes
+    const uint32_t per_cu = (64u * 1024u * 5u / 2u) / (lds + (uint32_t)(2 * 8 * 8 * sizeof(CrtInvLds)));
     const uint32_t cap = d.cus * (per_cu < 2u ? per_cu : 2u);
     const uint32_t grid = groups < cap ? groups : cap;
     crt30_mfma_kernel<4, 8, 28, 2><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
