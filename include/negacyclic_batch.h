@@ -86,6 +86,11 @@ typedef struct nc_server_stats {
 
 typedef struct nc_server nc_server;
 
+/* Creates the per-degree plans (autotuned at batch 1 and max_batch when
+ * config->autotune), lanes and scheduler thread, and returns once that
+ * thread has run every lane at every batch-size bucket (1, 2, 4, ..,
+ * max_batch) on zeros, so that no request pays for a kernel's or a
+ * stream's first use. */
 int nc_server_create(const nc_server_config* config, nc_server** out);
 /* Waits for every submitted request, then frees everything (null safe). */
 void nc_server_destroy(nc_server* server);
