@@ -266,7 +266,7 @@ def main() -> int:
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: products are independent and this build has no multi-GPU path "
+        print("bench.py measures one GPU only: products are independent and the library has no multi-GPU path "
               "(--gpus must be 1).", file=sys.stderr)
         return 2
 

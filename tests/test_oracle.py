@@ -90,7 +90,7 @@ def test_garner_matches_fast_crt(N, q868):
 
 
 # ---------------------------------------------------------------------------
-# closed forms (BASELINE edge inputs)
+# closed forms (edge inputs: all q - 1, zero, the X^N = -1 sign wrap, monomials)
 # ---------------------------------------------------------------------------
 
 def both(a, b, q):
