@@ -333,6 +333,24 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 template <class A, int R, int NE, bool MAY_TOP = true>
 __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, int lo, int log_n,
                                         const typename A::Twid* __restrict__ tw, const typename A::Prime& c) {
+    constexpr int MAXH = 1 << (R - 1);
+    // 32-bit words: every stage's twiddles are loaded before the first
+    // butterfly (a top stage's slot reads tw[1], in bounds and unused), so
+    // their latency hides behind the earlier stages; 64-bit twiddles would
+    // take too many registers and load stage by stage
+    constexpr bool kAhead = sizeof(typename A::T) == 4;
+    typename A::Twid ta[kAhead ? R : 1][MAXH];
+    if constexpr (kAhead) {
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int beta = lo + kk;
+            const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
+            const int nh = 1 << (R - 1 - kk);
+#pragma unroll
+            for (int h = 0; h < MAXH; ++h)
+                if (h < nh) ta[kk][h] = tw[base + h];
+        }
+    }
 #pragma unroll
     for (int kk = 0; kk < R; ++kk) {
         const int beta = lo + kk;
@@ -345,20 +363,24 @@ __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, in
             }
             continue;
         }
-        const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-        constexpr int MAXH = 1 << (R - 1);
         typename A::Twid t[MAXH];
-        const int nh = 1 << (R - 1 - kk);
+        if constexpr (kAhead) {
 #pragma unroll
-        for (int h = 0; h < MAXH; ++h)
-            if (h < nh) t[h] = tw[base + h];
+            for (int h = 0; h < MAXH; ++h) t[h] = ta[kk][h];
+        } else {
+            const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
+            const int nh = 1 << (R - 1 - kk);
+#pragma unroll
+            for (int h = 0; h < MAXH; ++h)
+                if (h < nh) t[h] = tw[base + h];
+        }
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int ep = e & ((1 << R) - 1);
             if (ep & (1 << kk)) continue;
             Bfly<A>::gs(v[e], v[e | (1 << kk)], t[ep >> (kk + 1)], c.p, c.p2);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (!kAhead) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -555,22 +577,27 @@ template <class A, int R>
 __device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A::T (&vb)[16], uint32_t gx0, int lo,
                                              int log_n, const typename A::Twid* __restrict__ tw, typename A::T p,
                                              typename A::T p2) {
+    // every stage's twiddles (1 + 2 + .. + 2^(R-1)) are loaded before the
+    // first butterfly, so their latency hides behind the earlier stages
+    constexpr int MAXH = 1 << (R - 1);
+    typename A::Twid t[R][MAXH];
 #pragma unroll
     for (int kk = R - 1; kk >= 0; --kk) {
         const int beta = lo + kk;
         const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-        constexpr int MAXH = 1 << (R - 1);
-        typename A::Twid t[MAXH];
         const int nh = 1 << (R - 1 - kk);
 #pragma unroll
         for (int h = 0; h < MAXH; ++h)
-            if (h < nh) t[h] = tw[base + h];
+            if (h < nh) t[kk][h] = tw[base + h];
+    }
+#pragma unroll
+    for (int kk = R - 1; kk >= 0; --kk) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int ep = e & ((1 << R) - 1);
             if (ep & (1 << kk)) continue;
-            Bfly<A>::ct(va[e], va[e | (1 << kk)], t[ep >> (kk + 1)], p, p2);
-            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], t[ep >> (kk + 1)], p, p2);
+            Bfly<A>::ct(va[e], va[e | (1 << kk)], t[kk][ep >> (kk + 1)], p, p2);
+            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], t[kk][ep >> (kk + 1)], p, p2);
         }
     }
 }
