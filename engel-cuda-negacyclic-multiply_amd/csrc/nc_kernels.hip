@@ -573,31 +573,63 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
 // padded layouts stay conflict-free at 8 bytes), the pointwise product needs
 // no parking, and the inverse transform runs on the product alone.
 // ---------------------------------------------------------------------------
-template <class A, int R>
-__device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A::T (&vb)[16], uint32_t gx0, int lo,
-                                             int log_n, const typename A::Twid* __restrict__ tw, typename A::T p,
-                                             typename A::T p2) {
-    // every stage's twiddles (1 + 2 + .. + 2^(R-1)) are loaded before the
-    // first butterfly, so their latency hides behind the earlier stages
-    constexpr int MAXH = 1 << (R - 1);
-    typename A::Twid t[R][MAXH];
+// One pass's twiddles (stage kk of R: 2^(R-1-kk) of them), loaded ahead of
+// the pass: before the LDS exchange that precedes it, so their latency
+// hides behind the exchange.  A top stage's slot reads tw[1] (in bounds,
+// unused).
+template <int R>
+struct PassTw30 {
+    Tw32 t[R][1 << (R - 1)];
+};
+
+template <int R>
+__device__ __forceinline__ void load_pass_tw(PassTw30<R>& pt, uint32_t gx0, int lo, int log_n,
+                                             const Tw32* __restrict__ tw) {
 #pragma unroll
     for (int kk = R - 1; kk >= 0; --kk) {
         const int beta = lo + kk;
         const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-        const int nh = 1 << (R - 1 - kk);
 #pragma unroll
-        for (int h = 0; h < MAXH; ++h)
-            if (h < nh) t[kk][h] = tw[base + h];
+        for (int h = 0; h < (1 << (R - 1 - kk)); ++h) pt.t[kk][h] = tw[base + h];
     }
+}
+
+template <class A, int R>
+__device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A::T (&vb)[16], const PassTw30<R>& pt,
+                                             typename A::T p, typename A::T p2) {
 #pragma unroll
     for (int kk = R - 1; kk >= 0; --kk) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int ep = e & ((1 << R) - 1);
             if (ep & (1 << kk)) continue;
-            Bfly<A>::ct(va[e], va[e | (1 << kk)], t[kk][ep >> (kk + 1)], p, p2);
-            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], t[kk][ep >> (kk + 1)], p, p2);
+            Bfly<A>::ct(va[e], va[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], p, p2);
+            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], p, p2);
+        }
+    }
+}
+
+// The inverse pass of 30-bit words on preloaded twiddles (MAY_TOP: as gs_pass).
+template <int R, int NE, bool MAY_TOP>
+__device__ __forceinline__ void gs_pass_tw(uint32_t (&v)[NE], const PassTw30<R>& pt, int lo, int log_n,
+                                           const PrimeConst32& c) {
+#pragma unroll
+    for (int kk = 0; kk < R; ++kk) {
+        const int beta = lo + kk;
+        if (MAY_TOP && kk == R - 1 && beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
+#pragma unroll
+            for (int e = 0; e < NE; ++e) {
+                const int ep = e & ((1 << R) - 1);
+                if (ep & (1 << kk)) continue;
+                Bfly<W30>::gs_last(v[e], v[e | (1 << kk)], c);
+            }
+            continue;
+        }
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int ep = e & ((1 << R) - 1);
+            if (ep & (1 << kk)) continue;
+            Bfly<W30>::gs(v[e], v[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], c.p, c.p2);
         }
     }
 }
@@ -660,13 +692,40 @@ __device__ __forceinline__ void exchange_pair(uint32_t (&va)[16], uint32_t (&vb)
     }
 }
 
-template <int LOGB, int M>
+// PF: each pass's twiddles are loaded before the exchange that precedes it
+// (the caller loads pt); otherwise at the start of the pass (pt is loaded
+// here).  Prefetching keeps one more pass's twiddles live across the
+// exchange, which only some block sizes have the registers for.
+template <int LOGB, int M, bool PF>
 __device__ __forceinline__ void forward_pair_from(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid,
-                                                  uint32_t gblk, int log_n, const Tw32* tw, uint32_t p, uint32_t p2) {
+                                                  uint32_t gblk, int log_n, const Tw32* tw, uint32_t p, uint32_t p2,
+                                                  PassTw30<PassMap<LOGB, M>::R>& pt) {
     using P = PassMap<LOGB, M>;
     if constexpr (M > 0) exchange_pair<LOGB, M - 1, M>(va, vb, lds, tid);
-    ct_pass_pair<W30, P::R>(va, vb, gblk | P::base(tid), P::LO, log_n, tw, p, p2);
-    if constexpr (M + 1 < InnerShape<LOGB>::NP) forward_pair_from<LOGB, M + 1>(va, vb, lds, tid, gblk, log_n, tw, p, p2);
+    if constexpr (!PF) load_pass_tw<P::R>(pt, gblk | P::base(tid), P::LO, log_n, tw);
+    ct_pass_pair<W30, P::R>(va, vb, pt, p, p2);
+    if constexpr (M + 1 < InnerShape<LOGB>::NP) {
+        using Q = PassMap<LOGB, M + 1>;
+        PassTw30<Q::R> nt;
+        if constexpr (PF) load_pass_tw<Q::R>(nt, gblk | Q::base(tid), Q::LO, log_n, tw);
+        forward_pair_from<LOGB, M + 1, PF>(va, vb, lds, tid, gblk, log_n, tw, p, p2, nt);
+    }
+}
+
+template <int LOGB, int M, bool PF>
+__device__ __forceinline__ void inverse30_from(uint32_t (&v)[16], uint32_t* lds, uint32_t tid, uint32_t gblk,
+                                               int log_n, const Tw32* tw, const PrimeConst32& c,
+                                               PassTw30<PassMap<LOGB, M>::R>& pt) {
+    using P = PassMap<LOGB, M>;
+    if constexpr (!PF) load_pass_tw<P::R>(pt, gblk | P::base(tid), P::LO, log_n, tw);
+    gs_pass_tw<P::R, 16, M == 0>(v, pt, P::LO, log_n, c);
+    if constexpr (M > 0) {
+        using Q = PassMap<LOGB, M - 1>;
+        PassTw30<Q::R> nt;
+        if constexpr (PF) load_pass_tw<Q::R>(nt, gblk | Q::base(tid), Q::LO, log_n, tw);
+        exchange<W30, LOGB, M, M - 1>(v, lds, tid);
+        inverse30_from<LOGB, M - 1, PF>(v, lds, tid, gblk, log_n, tw, c, nt);
+    }
 }
 
 template <int LOGB>
@@ -711,12 +770,18 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
         va[e] = ga0[IO::delta(e)];
         vb[e] = gb0[IO::delta(e)];
     }
+    constexpr bool PF = LOGB >= 10 && LOGB <= 12;  // the block sizes whose registers allow it
+    PassTw30<P0::R> tw0;
+    if constexpr (PF) load_pass_tw<P0::R>(tw0, gblk | P0::base(tid), P0::LO, (int)log_n, tf);
     if constexpr (kLinear) exchange_pair_maps<LinearMap<LOGB>, P0>(va, vb, lds2, tid);
-    forward_pair_from<LOGB, 0>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2);
+    forward_pair_from<LOGB, 0, PF>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2, tw0);
 #pragma unroll
     for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
+    using PL = PassMap<LOGB, LAST>;
+    PassTw30<PL::R> twl;
+    if constexpr (PF) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
     __syncthreads();  // the paired region is reused below with the one-operand layout
-    inverse_from<W30, LOGB, LAST>(va, lds1, tid, gblk, (int)log_n, ti, c);
+    inverse30_from<LOGB, LAST, PF>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
     if constexpr (kLinear) exchange_maps<P0, LinearMap<LOGB>>(va, lds1, tid);
     if (!active) return;
 #pragma unroll
