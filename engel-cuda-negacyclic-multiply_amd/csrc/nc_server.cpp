@@ -161,8 +161,9 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
 // the pipeline at every batch-size bucket (1, 2, 4, .., cap: each bucket's
 // tuning may use kernels nothing has launched yet, and a kernel's first
 // launch in a process is slow), the copy out, all on the lane's own stream.
-// Without it the first ~0.5 s of a process served at 6000 requests/s ran
-// behind (p99 ~40 ms against ~4 ms later).
+// Measured at 6000 requests/s: without it the first ~0.6 s of a process ran
+// behind (p99 ~40 ms, against ~4 ms later); with it the first ~0.1 s still
+// does.
 int warm_lanes(nc_server* sv) {
     const uint32_t L = sv->cfg.L;
     size_t max_words = 0;
@@ -196,6 +197,10 @@ int warm_lanes(nc_server* sv) {
         }
         if (rc != NC_OK) break;
     }
+    // no copy may still use h when it is freed (an error above can leave
+    // work queued on a lane)
+    for (auto& kv : sv->deg)
+        for (Lane& ln : kv.second.lanes) (void)hipStreamSynchronize(ln.s);
     (void)hipHostFree(h);
     return rc;
 }
