@@ -399,6 +399,17 @@ def main() -> int:
         "unit": "GB/s", "frac": byts[dom] / dom_s / HBM_PEAK, "traffic": traffic,
         "bytes_per_launch": byts[dom],
     }
+    # every stage against both bounds: compulsory HBM bytes and convention
+    # multiplies per launch over its event-timed duration
+    stage_roofline = {}
+    for st, ms in stage_ms.items():
+        if ms <= 0.0:
+            continue
+        sec = ms * 1e-3
+        stage_roofline[st] = {
+            "ms": ms, "hbm_GBps": byts[st] / sec / 1e9, "hbm_frac": byts[st] / sec / HBM_PEAK,
+            "mul32_frac": work[st] / sec / peak_mul,
+        }
     W = N * (k * (6 * L + 15 * (N.bit_length() - 1) + 29) + 406)
     step_total_ms = sum(stage_ms.values())
 
@@ -439,6 +450,7 @@ def main() -> int:
         "stage_ms": stage_ms,
         "roofline": roofline,
         "roofline_hbm": roofline_hbm,
+        "stage_roofline": stage_roofline,
         "w_convention": {
             "W_per_product": W,
             "whole_path_mul32_per_s": batched_pps * W,
