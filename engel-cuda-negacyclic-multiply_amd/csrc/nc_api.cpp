@@ -601,6 +601,28 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
         cands.push_back({30u, mfma_default(p)});
     }
     cands.push_back({62u, 0u});
+    // runs of choice c (after one warm-up run): their sum, or their minimum
+    auto time_choice = [&](const Choice& c, int runs, bool min_of_runs = false) {
+        float acc = min_of_runs ? -1.f : 0.f;
+        for (int rep = 0; rep <= runs && e == hipSuccess; ++rep) {  // rep 0 warms up
+            e = hipEventRecord(t0, s);
+            if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, c) != NC_OK)
+                e = hipErrorLaunchFailure;
+            if (e == hipSuccess) e = hipEventRecord(t1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+            if (rep == 0) continue;
+            if (!min_of_runs) acc += t;
+            else if (acc < 0.f || t < acc) acc = t;
+        }
+        return acc;
+    };
+    struct Timed {
+        float ms;
+        Choice c;
+    };
+    std::vector<Timed> timed;
     std::vector<uint32_t> chunk_opts = {1u};
     if (batch >= 8) chunk_opts.push_back(2u);
     if (batch >= 32) chunk_opts.push_back(4u);
@@ -614,23 +636,27 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
             c.mfma = cd.mfma;
             c.log_blocks = lb;
             c.chunks = ch;
-            float ms = 0.f;
-            for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {  // rep 0 warms up
-                e = hipEventRecord(t0, s);
-                if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, c) != NC_OK)
-                    e = hipErrorLaunchFailure;
-                if (e == hipSuccess) e = hipEventRecord(t1, s);
-                if (e == hipSuccess) e = hipEventSynchronize(t1);
-                float t = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
-                if (rep > 0) ms += t;
-            }
-            if (e == hipSuccess && (best_ms < 0.f || ms < best_ms)) {
-                best_ms = ms;
-                best = c;
-            }
+            const float ms = time_choice(c, 2);
+            if (e == hipSuccess) timed.push_back({ms, c});
         }
     }
+    // The quick pass above times each candidate twice; runs on one device
+    // vary by several percent, as much as the leaders differ.  The three
+    // fastest are timed again, in turn, over three rounds, and the lowest
+    // single run decides.
+    std::sort(timed.begin(), timed.end(), [](const Timed& x, const Timed& y) { return x.ms < y.ms; });
+    const size_t top = std::min<size_t>(3, timed.size());
+    std::vector<float> best_run(top, -1.f);
+    for (int round = 0; round < 3 && e == hipSuccess; ++round)
+        for (size_t i = 0; i < top && e == hipSuccess; ++i) {
+            const float ms = time_choice(timed[i].c, 3, /*min_of_runs=*/true);
+            if (e == hipSuccess && (best_run[i] < 0.f || ms < best_run[i])) best_run[i] = ms;
+        }
+    for (size_t i = 0; i < top; ++i)
+        if (best_run[i] >= 0.f && (best_ms < 0.f || best_run[i] < best_ms)) {
+            best_ms = best_run[i];
+            best = timed[i].c;
+        }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     (void)hipStreamSynchronize(s);
