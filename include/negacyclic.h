@@ -130,8 +130,10 @@ int nc_plan_set_chunks(nc_plan* plan, uint32_t chunks);
 int nc_plan_set_mfma(nc_plan* plan, int mode);
 
 /* Autotuner: times every (prime width, NTT split, matrix-core paths, chunks) for `batch` products on
- * `stream` (zero inputs; the kernels' work does not depend on the data) and
- * keeps the fastest for batches of that size bucket (ceil(log2 batch)).
+ * `stream` (zero inputs; the kernels' work does not depend on the data),
+ * twice each, then the three fastest again in turn over three rounds, and
+ * keeps the one with the lowest single run for batches of that size bucket
+ * (ceil(log2 batch)).
  * Clears a setting forced by nc_plan_set_tuning.  Output bits never depend
  * on it.  (The reference has a single fixed schedule, solve.cu:1006-1038.) */
 int nc_plan_autotune(nc_plan* plan, uint32_t batch, void* stream, uint32_t* chosen_width,
