@@ -169,7 +169,7 @@ def test_stage_product_residues_and_crt_30(gpu, q868, mfma):
 
 
 @pytest.mark.parametrize("width", WIDTHS)
-@pytest.mark.parametrize("N", [16, 64, 256, 1024, 16384, 65536])
+@pytest.mark.parametrize("N", [16, 64, 256, 1024, 2048, 8192, 16384, 32768, 65536])
 def test_tuning_invariance(gpu, N, width, q868):
     """Every accepted NTT split gives the same bits as the oracle (split 2
     with the matrix-core CRT runs the last inverse stages inside the CRT;
