@@ -770,7 +770,11 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
         va[e] = ga0[IO::delta(e)];
         vb[e] = gb0[IO::delta(e)];
     }
-    constexpr bool PF = LOGB >= 10 && LOGB <= 12;  // the block sizes whose registers allow it
+    // the block sizes where the prefetch measured faster: the forward
+    // transforms hold 32 values and have the registers for it only up to
+    // 2^12; the inverse (16 values) also gains at 2^14 (5 %), not at 2^13
+    constexpr bool PF = LOGB >= 10 && LOGB <= 12;
+    constexpr bool PFI = PF || LOGB == 14;
     PassTw30<P0::R> tw0;
     if constexpr (PF) load_pass_tw<P0::R>(tw0, gblk | P0::base(tid), P0::LO, (int)log_n, tf);
     if constexpr (kLinear) exchange_pair_maps<LinearMap<LOGB>, P0>(va, vb, lds2, tid);
@@ -779,9 +783,9 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
     using PL = PassMap<LOGB, LAST>;
     PassTw30<PL::R> twl;
-    if constexpr (PF) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
+    if constexpr (PFI) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
     __syncthreads();  // the paired region is reused below with the one-operand layout
-    inverse30_from<LOGB, LAST, PF>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
+    inverse30_from<LOGB, LAST, PFI>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
     if constexpr (kLinear) exchange_maps<P0, LinearMap<LOGB>>(va, lds1, tid);
     if (!active) return;
 #pragma unroll
