@@ -299,9 +299,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
 
 // ===========================================================================
 // NTT passes on registers.  A pass handles R consecutive index bits
-// [lo, lo + R) of the values a thread holds.  Register e = (eq << R) | ep:
-// ep enumerates the pass bits, eq (when R < 4) enumerates 4 - R further bits
-// that lie below the pass.  gx0 is the global row index of register 0.
+// [lo, lo + R) of the NE values a thread holds; register e sits at global
+// row index gx0 + (e << lo).  When NE > 2^R the bits of e above R are spare:
+// they index further groups the pass transforms side by side (only a pass
+// with lo = 0 has them, and the low bits of gx0 are then zero), so stage kk
+// of register e uses twiddle (e >> (kk + 1)) of the stage's run.
 // ===========================================================================
 template <class A, int R, int NE>
 __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, int lo, int log_n,
@@ -311,17 +313,16 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
     for (int kk = R - 1; kk >= 0; --kk) {
         const int beta = lo + kk;
         const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-        constexpr int MAXH = 1 << (R - 1);
+        constexpr int MAXH = NE / 2;
         typename A::Twid t[MAXH];
-        const int nh = 1 << (R - 1 - kk);
+        const int nh = NE >> (kk + 1);
 #pragma unroll
         for (int h = 0; h < MAXH; ++h)
             if (h < nh) t[h] = tw[base + h];
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const int ep = e & ((1 << R) - 1);
-            if (ep & (1 << kk)) continue;
-            Bfly<A>::ct(v[e], v[e | (1 << kk)], t[ep >> (kk + 1)], p, p2);
+            if (e & (1 << kk)) continue;
+            Bfly<A>::ct(v[e], v[e | (1 << kk)], t[e >> (kk + 1)], p, p2);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -333,7 +334,7 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 template <class A, int R, int NE, bool MAY_TOP = true>
 __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, int lo, int log_n,
                                         const typename A::Twid* __restrict__ tw, const typename A::Prime& c) {
-    constexpr int MAXH = 1 << (R - 1);
+    constexpr int MAXH = NE / 2;
     // 32-bit words: every stage's twiddles are loaded before the first
     // butterfly (a top stage's slot reads tw[1], in bounds and unused), so
     // their latency hides behind the earlier stages; 64-bit twiddles would
@@ -345,7 +346,7 @@ __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, in
         for (int kk = 0; kk < R; ++kk) {
             const int beta = lo + kk;
             const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-            const int nh = 1 << (R - 1 - kk);
+            const int nh = NE >> (kk + 1);
 #pragma unroll
             for (int h = 0; h < MAXH; ++h)
                 if (h < nh) ta[kk][h] = tw[base + h];
@@ -357,8 +358,7 @@ __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, in
         if (MAY_TOP && kk == R - 1 && beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
-                const int ep = e & ((1 << R) - 1);
-                if (ep & (1 << kk)) continue;
+                if (e & (1 << kk)) continue;
                 Bfly<A>::gs_last(v[e], v[e | (1 << kk)], c);
             }
             continue;
@@ -369,16 +369,15 @@ __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, in
             for (int h = 0; h < MAXH; ++h) t[h] = ta[kk][h];
         } else {
             const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
-            const int nh = 1 << (R - 1 - kk);
+            const int nh = NE >> (kk + 1);
 #pragma unroll
             for (int h = 0; h < MAXH; ++h)
                 if (h < nh) t[h] = tw[base + h];
         }
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const int ep = e & ((1 << R) - 1);
-            if (ep & (1 << kk)) continue;
-            Bfly<A>::gs(v[e], v[e | (1 << kk)], t[ep >> (kk + 1)], c.p, c.p2);
+            if (e & (1 << kk)) continue;
+            Bfly<A>::gs(v[e], v[e | (1 << kk)], t[e >> (kk + 1)], c.p, c.p2);
         }
         if constexpr (!kAhead) __builtin_amdgcn_sched_barrier(0);
     }
@@ -390,37 +389,31 @@ __device__ __forceinline__ void gs_pass(typename A::T (&v)[NE], uint32_t gx0, in
 template <int LOGB>
 struct InnerShape {
     static constexpr int TPB = 1 << (LOGB - 4);             // threads per block (16 values each)
-    static constexpr int R0 = LOGB - 4 * ((LOGB - 1) / 4);  // bits of pass 0 (1..4)
-    static constexpr int NP = 1 + (LOGB - R0) / 4;          // passes
+    static constexpr int RL = LOGB - 4 * ((LOGB - 1) / 4);  // bits of the last pass (1..4)
+    static constexpr int NP = 1 + (LOGB - RL) / 4;          // passes
     static constexpr int BPW = TPB >= 256 ? 1 : 256 / TPB;  // blocks per workgroup
     static constexpr int WG = BPW * TPB;
     static constexpr int PADB = (1 << LOGB) + (1 << (LOGB - 4));  // one pad word per 16
 };
 
 // Layout of pass m: register e of thread tid holds local index
-//   x = base(tid) + delta(e)
-// where base and delta occupy disjoint bits.  Pass 0 covers bits
-// [LOGB - R0, LOGB) with 4 - R0 more bits [0, 4 - R0) below them; pass m >= 1
-// covers bits [LO, LO + 4).  LDS rows are padded by one word per 16
-// (pad(x) = x + (x >> 4)); pad is additive over disjoint bit fields, so every
-// per-register LDS and global offset below is a compile-time constant added
-// to one per-thread base.
+//   x = base(tid) + delta(e),  delta(e) = e << LO,
+// where base fills the other bits.  Pass m < NP - 1 covers the four bits
+// [LO, LO + 4) counted down from the top (pass 0: the top four, so x = tid +
+// e TPB -- loads and stores in it are lane-contiguous); the last pass covers
+// [0, RL), RL <= 4, its registers also spanning bits [RL, 4) side by side.
+// LDS rows are padded by one word per 16 (pad(x) = x + (x >> 4)); pad is
+// additive over disjoint bit fields, so every per-register LDS and global
+// offset below is a compile-time constant added to one per-thread base.
 template <int LOGB, int M>
 struct PassMap {
     using S = InnerShape<LOGB>;
-    static constexpr int LO = LOGB - S::R0 - 4 * M;
-    static constexpr int R = M == 0 ? S::R0 : 4;
+    static constexpr int LO = M == S::NP - 1 ? 0 : LOGB - 4 * (M + 1);
+    static constexpr int R = M == S::NP - 1 ? S::RL : 4;
     static constexpr uint32_t base(uint32_t tid) {
-        if constexpr (M == 0) return tid << (4 - S::R0);
-        else return (tid & ((1u << LO) - 1)) | ((tid >> LO) << (LO + 4));
+        return (tid & ((1u << LO) - 1)) | ((tid >> LO) << (LO + 4));
     }
-    static constexpr uint32_t delta(int e) {
-        if constexpr (M == 0) {
-            return ((uint32_t)e >> S::R0) | (((uint32_t)e & ((1u << S::R0) - 1)) << LO);
-        } else {
-            return (uint32_t)e << LO;
-        }
-    }
+    static constexpr uint32_t delta(int e) { return (uint32_t)e << LO; }
     // index bits that one wave's lanes and registers span (base and delta
     // are bit permutations of tid and e)
     static constexpr uint32_t wave_bits() {
@@ -573,13 +566,14 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
 // padded layouts stay conflict-free at 8 bytes), the pointwise product needs
 // no parking, and the inverse transform runs on the product alone.
 // ---------------------------------------------------------------------------
-// One pass's twiddles (stage kk of R: 2^(R-1-kk) of them), loaded ahead of
-// the pass: before the LDS exchange that precedes it, so their latency
-// hides behind the exchange.  A top stage's slot reads tw[1] (in bounds,
-// unused).
+// One pass's twiddles (stage kk: 8 >> kk of them for the 16 registers; a
+// pass of R < 4 bits has spare register bits above its own, see ct_pass),
+// loaded ahead of the pass: before the LDS exchange that precedes it, so
+// their latency hides behind the exchange.  A top stage's slot reads tw[1]
+// (in bounds, unused).
 template <int R>
 struct PassTw30 {
-    Tw32 t[R][1 << (R - 1)];
+    Tw32 t[R][8];
 };
 
 template <int R>
@@ -590,7 +584,7 @@ __device__ __forceinline__ void load_pass_tw(PassTw30<R>& pt, uint32_t gx0, int 
         const int beta = lo + kk;
         const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
 #pragma unroll
-        for (int h = 0; h < (1 << (R - 1 - kk)); ++h) pt.t[kk][h] = tw[base + h];
+        for (int h = 0; h < (8 >> kk); ++h) pt.t[kk][h] = tw[base + h];
     }
 }
 
@@ -601,10 +595,9 @@ __device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A
     for (int kk = R - 1; kk >= 0; --kk) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            const int ep = e & ((1 << R) - 1);
-            if (ep & (1 << kk)) continue;
-            Bfly<A>::ct(va[e], va[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], p, p2);
-            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], p, p2);
+            if (e & (1 << kk)) continue;
+            Bfly<A>::ct(va[e], va[e | (1 << kk)], pt.t[kk][e >> (kk + 1)], p, p2);
+            Bfly<A>::ct(vb[e], vb[e | (1 << kk)], pt.t[kk][e >> (kk + 1)], p, p2);
         }
     }
 }
@@ -619,58 +612,17 @@ __device__ __forceinline__ void gs_pass_tw(uint32_t (&v)[NE], const PassTw30<R>&
         if (MAY_TOP && kk == R - 1 && beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
 #pragma unroll
             for (int e = 0; e < NE; ++e) {
-                const int ep = e & ((1 << R) - 1);
-                if (ep & (1 << kk)) continue;
+                if (e & (1 << kk)) continue;
                 Bfly<W30>::gs_last(v[e], v[e | (1 << kk)], c);
             }
             continue;
         }
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
-            const int ep = e & ((1 << R) - 1);
-            if (ep & (1 << kk)) continue;
-            Bfly<W30>::gs(v[e], v[e | (1 << kk)], pt.t[kk][ep >> (kk + 1)], c.p, c.p2);
+            if (e & (1 << kk)) continue;
+            Bfly<W30>::gs(v[e], v[e | (1 << kk)], pt.t[kk][e >> (kk + 1)], c.p, c.p2);
         }
     }
-}
-
-// Lane-contiguous layout x = tid + e TPB: global loads and stores in it are
-// fully coalesced.  Pass 0's layout is lane-contiguous only when its pass has
-// 4 bits (R0 = 4); otherwise the kernel loads and stores in this layout and
-// converts through one more LDS exchange.
-template <int LOGB>
-struct LinearMap {
-    using S = InnerShape<LOGB>;
-    __device__ __forceinline__ static uint32_t base(uint32_t tid) { return tid; }
-    static constexpr uint32_t delta(int e) { return (uint32_t)e * S::TPB; }
-};
-
-template <class F, class G>
-__device__ __forceinline__ void exchange_pair_maps(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid) {
-    uint2* w = lds + lds_pad(F::base(tid));
-    uint2* r = lds + lds_pad(G::base(tid));
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = make_uint2(va[e], vb[e]);
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const uint2 x = r[lds_pad(G::delta(e))];
-        va[e] = x.x;
-        vb[e] = x.y;
-    }
-}
-
-template <class F, class G>
-__device__ __forceinline__ void exchange_maps(uint32_t (&v)[16], uint32_t* lds, uint32_t tid) {
-    uint32_t* w = lds + lds_pad(F::base(tid));
-    uint32_t* r = lds + lds_pad(G::base(tid));
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 16; ++e) w[lds_pad(F::delta(e))] = v[e];
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = r[lds_pad(G::delta(e))];
 }
 
 template <int LOGB, int FROM, int TO>
@@ -759,16 +711,14 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     const Tw32* tf = twf + (size_t)j * N;
     const Tw32* ti = twi + (size_t)j * N;
     const bool final_out = (nblk_log == 0);
-    using P0 = PassMap<LOGB, 0>;
-    constexpr bool kLinear = S::R0 < 4;  // pass 0 not lane-contiguous: go through LinearMap
-    using IO = typename std::conditional<kLinear, LinearMap<LOGB>, P0>::type;
-    T* ga0 = ra + off + IO::base(tid);
-    const T* gb0 = rb + off + IO::base(tid);
+    using P0 = PassMap<LOGB, 0>;  // lane-contiguous: x = tid + e TPB
+    T* ga0 = ra + off + P0::base(tid);
+    const T* gb0 = rb + off + P0::base(tid);
     T va[16], vb[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        va[e] = ga0[IO::delta(e)];
-        vb[e] = gb0[IO::delta(e)];
+        va[e] = ga0[P0::delta(e)];
+        vb[e] = gb0[P0::delta(e)];
     }
     // the block sizes where the prefetch measured faster: the forward
     // transforms hold 32 values and have the registers for it only up to
@@ -777,7 +727,6 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     constexpr bool PFI = PF || LOGB == 14;
     PassTw30<P0::R> tw0;
     if constexpr (PF) load_pass_tw<P0::R>(tw0, gblk | P0::base(tid), P0::LO, (int)log_n, tf);
-    if constexpr (kLinear) exchange_pair_maps<LinearMap<LOGB>, P0>(va, vb, lds2, tid);
     forward_pair_from<LOGB, 0, PF>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2, tw0);
 #pragma unroll
     for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
@@ -786,10 +735,9 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     if constexpr (PFI) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
     __syncthreads();  // the paired region is reused below with the one-operand layout
     inverse30_from<LOGB, LAST, PFI>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
-    if constexpr (kLinear) exchange_maps<P0, LinearMap<LOGB>>(va, lds1, tid);
     if (!active) return;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) ga0[IO::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
+    for (int e = 0; e < 16; ++e) ga0[P0::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
 }
 
 // ---------------------------------------------------------------------------
