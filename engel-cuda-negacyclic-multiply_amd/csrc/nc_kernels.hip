@@ -720,11 +720,10 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
         va[e] = ga0[P0::delta(e)];
         vb[e] = gb0[P0::delta(e)];
     }
-    // the block sizes where the prefetch measured faster: the forward
-    // transforms hold 32 values and have the registers for it only up to
-    // 2^12; the inverse (16 values) also gains at 2^14 (5 %), not at 2^13
-    constexpr bool PF = LOGB >= 10 && LOGB <= 12;
-    constexpr bool PFI = PF || LOGB == 14;
+    // every pass's twiddles load before the exchange that precedes it from
+    // blocks of 2^10 up (it measured faster or the same there and fits in
+    // 128 VGPRs); smaller blocks load at the start of each pass
+    constexpr bool PF = LOGB >= 10;
     PassTw30<P0::R> tw0;
     if constexpr (PF) load_pass_tw<P0::R>(tw0, gblk | P0::base(tid), P0::LO, (int)log_n, tf);
     forward_pair_from<LOGB, 0, PF>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2, tw0);
@@ -732,9 +731,9 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
     using PL = PassMap<LOGB, LAST>;
     PassTw30<PL::R> twl;
-    if constexpr (PFI) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
+    if constexpr (PF) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
     __syncthreads();  // the paired region is reused below with the one-operand layout
-    inverse30_from<LOGB, LAST, PFI>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
+    inverse30_from<LOGB, LAST, PF>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
     if (!active) return;
 #pragma unroll
     for (int e = 0; e < 16; ++e) ga0[P0::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
