@@ -178,7 +178,11 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 // are then 4 / 2^S1 column groups of 2^S1 tiles spaced N / 2^S1 apart, so each
 // lane holds, per prime, the 2^S1 values those stages combine.  Output is
 // then lazy in [0, 4p), exactly what ntt_outer_fwd_kernel<W30, S1> leaves.
-template <int S1>
+// WS > 1 (launches with few work items): WS waves share each chunk, wave w
+// of a group taking prime tiles [w T / WS, (w + 1) T / WS) of the T tiles,
+// so each wave's serial chain is WS times shorter; a work item then holds
+// 4 / WS chunks.
+template <int S1, int WS = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void decompose30_mfma_kernel(
     const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
     uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int32_t* __restrict__ bias,
@@ -210,7 +214,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     __syncthreads();
     const uint32_t span = N >> S1;                      // columns of one spread tile group
     const uint32_t chunks = span / 32;                  // 32-column chunks per polynomial
-    const uint32_t wgs_per_poly = (chunks + 4 * NG - 1) / (4 * NG);
+    static_assert(WS == 1 || WS == 2 || WS == 4, "waves per chunk");
+    constexpr uint32_t kChunksPerItem = 4 * NG / WS;
+    const uint32_t wgs_per_poly = (chunks + kChunksPerItem - 1) / kChunksPerItem;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 31u, half = lane >> 5;
     const uint32_t nbytes = 4 * L;
@@ -218,13 +224,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     // every work item it takes (work item = one polynomial's 4 NG chunks)
     for (uint32_t item = blockIdx.x; item < wgs_per_poly * (na + nb); item += gridDim.x) {
     const uint32_t poly = item / wgs_per_poly;
-    const uint32_t chunk0 = (item - poly * wgs_per_poly) * (4 * NG);
+    const uint32_t chunk0 = (item - poly * wgs_per_poly) * kChunksPerItem;
     const bool is_a = poly < na;
     const uint32_t pi = is_a ? poly : poly - na;
     const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
     uint32_t* dst = (is_a ? res_a : res_b) + (size_t)pi * k * N;
     for (int g = 0; g < NG; ++g) {
-        const uint32_t ch = chunk0 + wave * NG + g;
+        const uint32_t ch = chunk0 + (wave / WS) * NG + g;
         if (ch >= chunks) break;  // wave-uniform; no barrier in the item loop
         const uint32_t c0 = ch * 32 + col;  // this lane's column; tile e holds coefficient c0 + e span
         nc_v4i bf[G][kMfmaK / 32];
@@ -242,7 +248,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                 bf[e][ks] = *reinterpret_cast<const nc_v4i*>(cp + offc) ^ (int)0x80808080;  // bytes a_m - 128
             }
         }
-        for (uint32_t t = 0; t < tiles; ++t) {
+        uint32_t t_lo = 0, t_hi = tiles;
+        if constexpr (WS > 1) {
+            t_lo = (wave % WS) * tiles / WS;
+            t_hi = (wave % WS + 1) * tiles / WS;
+        }
+        for (uint32_t t = t_lo; t < t_hi; ++t) {
             uint32_t v[4][G];  // [h][e]
             // 2^16 in a register the compiler cannot see through, so that
             // lo + 2^16 hi is one v_mad_i64_i32 rather than 64-bit shifts and adds
@@ -1637,24 +1648,28 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
                                                                d.w62.k);
     } else if (mfma && d.w30.mfma_tab && d.N >= 32) {
         constexpr uint32_t kPerWg = 4 * kMfmaCoeffTiles * 32;
-        const uint32_t items = ((d.N + kPerWg - 1) / kPerWg) * (na + nb);
+        const uint32_t items1 = ((d.N + kPerWg - 1) / kPerWg) * (na + nb);
+        // fewer work items than CUs (or than two per CU): 4 (2) waves per chunk
+        const uint32_t ws = items1 < d.cus ? 4u : items1 < 2 * d.cus ? 2u : 1u;
+        const uint32_t items = items1 * ws;
         const dim3 g(items < 3 * d.cus ? items : 3 * d.cus);  // persistent: three workgroups per CU
         uint32_t* ra = static_cast<uint32_t*>(res_a);
         uint32_t* rb = static_cast<uint32_t*>(res_b);
         const DevSet30& w = d.w30;
-        switch (fuse_s1) {
-            case 0:
-                decompose30_mfma_kernel<0><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
-                break;
-            case 1:
-                decompose30_mfma_kernel<1><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
-                break;
-            case 2:
-                decompose30_mfma_kernel<2><<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd,
-                                                             d.N, d.L, w.k, w.mfma_tiles, nb);
-                break;
+        auto go = [&](auto kern) {
+            kern<<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd, d.N, d.L, w.k,
+                                   w.mfma_tiles, nb);
+        };
+        switch (fuse_s1 * 3 + (ws == 1 ? 0 : ws == 2 ? 1 : 2)) {
+            case 0: go(decompose30_mfma_kernel<0, 1>); break;
+            case 1: go(decompose30_mfma_kernel<0, 2>); break;
+            case 2: go(decompose30_mfma_kernel<0, 4>); break;
+            case 3: go(decompose30_mfma_kernel<1, 1>); break;
+            case 4: go(decompose30_mfma_kernel<1, 2>); break;
+            case 5: go(decompose30_mfma_kernel<1, 4>); break;
+            case 6: go(decompose30_mfma_kernel<2, 1>); break;
+            case 7: go(decompose30_mfma_kernel<2, 2>); break;
+            case 8: go(decompose30_mfma_kernel<2, 4>); break;
             default: return hipErrorInvalidValue;
         }
     } else {
