@@ -7,6 +7,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/negacyclic.h"
 #include "nc_internal.hpp"
@@ -765,6 +766,7 @@ int nc_stage_crt(nc_plan* p, uint32_t width, const void* d_res, uint32_t* d_c, u
 int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
                       uint32_t reps, void* stream, float* ms) {
     if (!p || !d_a || !d_b || !d_c || !ms || reps == 0) return fail(NC_ERR_INVALID, "bad argument");
+    if (reps > 4096) return fail(NC_ERR_INVALID, "reps above 4096");
     int rc = check_batch(p, batch);
     if (rc) return rc;
     rc = grow_workspace(p, batch);
@@ -774,37 +776,46 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     const int w = (int)c.width;
     const bool dm = (c.mfma & kMfmaDecompose) != 0;
     const bool fuse = nc::decompose_fuses_outer(p->dev, w, dm, c.log_blocks);
-    hipEvent_t ev[6];
-    for (int i = 0; i < 6; ++i) NC_HIP(hipEventCreate(&ev[i]), "event");
-    double acc[5] = {0, 0, 0, 0, 0};
+    // reps run back to back, as a batched step's launches do; events are read
+    // after the last one
+    std::vector<hipEvent_t> ev(6 * (size_t)reps, nullptr);
     int status = NC_OK;
-    for (uint32_t r = 0; r < reps && status == NC_OK; ++r) {
-        hipError_t e = hipEventRecord(ev[0], s);
+    for (size_t i = 0; i < ev.size() && status == NC_OK; ++i) {
+        const hipError_t e = hipEventCreate(&ev[i]);
+        if (e != hipSuccess) status = hip_fail(e, "event");
+    }
+    double acc[5] = {0, 0, 0, 0, 0};
+    const bool inv = inv_in_crt(p, c);  // then the CRT time includes the last inverse stages
+    hipError_t e = hipSuccess;
+    for (uint32_t r = 0; r < reps && status == NC_OK && e == hipSuccess; ++r) {
+        hipEvent_t* er = &ev[6 * (size_t)r];
+        e = hipEventRecord(er[0], s);
         if (e == hipSuccess)
             e = nc::launch_decompose(p->dev, w, d_a, p->d_res_a, batch, d_b, p->d_res_b, batch, s, dm,
                                      fuse ? c.log_blocks : 0u);
-        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventRecord(er[1], s);
         if (e == hipSuccess && !fuse)
             e = nc::launch_ntt_outer_fwd(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+        if (e == hipSuccess) e = hipEventRecord(er[2], s);
         if (e == hipSuccess) e = nc::launch_ntt_inner(p->dev, w, p->d_res_a, p->d_res_b, batch, c.log_blocks, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[3], s);
-        const bool inv = inv_in_crt(p, c);  // then the CRT time includes the last inverse stages
+        if (e == hipSuccess) e = hipEventRecord(er[3], s);
         if (e == hipSuccess && !inv) e = nc::launch_ntt_outer_inv(p->dev, w, p->d_res_a, batch, c.log_blocks, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[4], s);
+        if (e == hipSuccess) e = hipEventRecord(er[4], s);
         if (e == hipSuccess)
             e = inv ? nc::launch_crt_inv2(p->dev, static_cast<const uint32_t*>(p->d_res_a), d_c, batch, s)
                     : nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0);
-        if (e == hipSuccess) e = hipEventRecord(ev[5], s);
-        if (e == hipSuccess) e = hipEventSynchronize(ev[5]);
+        if (e == hipSuccess) e = hipEventRecord(er[5], s);
+    }
+    if (status == NC_OK && e == hipSuccess) e = hipStreamSynchronize(s);
+    for (uint32_t r = 0; r < reps && status == NC_OK && e == hipSuccess; ++r)
         for (int i = 0; i < 5 && e == hipSuccess; ++i) {
             float t = 0;
-            e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+            e = hipEventElapsedTime(&t, ev[6 * (size_t)r + i], ev[6 * (size_t)r + i + 1]);
             acc[i] += t;
         }
-        if (e != hipSuccess) status = hip_fail(e, "profile");
-    }
-    for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
+    if (status == NC_OK && e != hipSuccess) status = hip_fail(e, "profile");
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
     if (status) return status;
     const bool outer = p->host.log_n >= 4 && c.log_blocks > 0;
     for (int i = 0; i < 5; ++i) ms[i] = (float)(acc[i] / reps);

@@ -175,9 +175,10 @@ int nc_stage_product_residues(nc_plan* plan, uint32_t width, const uint32_t* d_a
 int nc_stage_crt(nc_plan* plan, uint32_t width, const void* d_res, uint32_t* d_c, uint32_t batch, void* stream);
 
 /* ---- diagnostics -------------------------------------------------------
- * nc_profile_stages runs the batched pipeline `reps` times on `stream` and
- * returns the mean duration in milliseconds of each kernel launch, measured
- * with HIP events recorded on that stream between the launches:
+ * nc_profile_stages runs the batched pipeline `reps` times (at most 4096)
+ * back to back on `stream` and returns the mean duration in milliseconds of
+ * each kernel launch, measured with HIP events recorded on that stream
+ * between the launches:
  *   ms[0] decompose, ms[1] NTT outer forward, ms[2] NTT inner (forward,
  *   pointwise, inverse), ms[3] NTT outer inverse, ms[4] CRT.
  * A stage the plan's split does not launch reports 0, and so does a stage

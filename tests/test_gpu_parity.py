@@ -234,6 +234,35 @@ def test_chunked_batch(gpu, width, chunks, q868):
                 assert np.array_equal(got[i], orc.multiply(a[i], b[i])), (rep, i)
 
 
+@pytest.mark.parametrize("width,split", [(62, None), (30, 2), (30, 0)])
+def test_profile_stages(gpu, width, split, q868):
+    """The stage profiler runs the real pipeline back to back: its last
+    repetition leaves the products in c, every stage it reports is positive,
+    and stages that run inside a neighbour report 0."""
+    N, B = 2048, 3
+    say("profile", width, split)
+    a = np.stack([ncmul.gen_uniform(500 + 2 * i, N, q868) for i in range(B)])
+    b = np.stack([ncmul.gen_uniform(501 + 2 * i, N, q868) for i in range(B)])
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_tuning(width, split)
+        d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+        d_c = ncmul.empty_device_u32((B, N, 28))
+        ms = plan.profile_stages(d_a, d_b, d_c, B, reps=7)
+        got = ncmul.from_device_u32(d_c)
+        for i in range(B):
+            assert np.array_equal(got[i], orc.multiply(a[i], b[i])), i
+        assert set(ms) == {"decompose", "ntt_outer_fwd", "ntt_inner", "ntt_outer_inv", "crt"}
+        assert ms["decompose"] > 0 and ms["ntt_inner"] > 0 and ms["crt"] > 0
+        assert all(v >= 0 for v in ms.values())
+        if split == 0:  # one block per prime: no outer stages at all
+            assert ms["ntt_outer_fwd"] == 0 and ms["ntt_outer_inv"] == 0
+        with pytest.raises(ncmul.NcError):
+            plan.profile_stages(d_a, d_b, d_c, B, reps=0)
+        with pytest.raises(ncmul.NcError):
+            plan.profile_stages(d_a, d_b, d_c, B, reps=4097)
+
+
 def test_autotune_keeps_bits(gpu, q868):
     N = 4096
     a, b = ncmul.gen_uniform(61, N, q868), ncmul.gen_uniform(62, N, q868)
