@@ -234,6 +234,31 @@ def test_chunked_batch(gpu, width, chunks, q868):
                 assert np.array_equal(got[i], orc.multiply(a[i], b[i])), (rep, i)
 
 
+def test_decompose_wave_split(gpu, q868):
+    """Batch sizes on both sides of the decomposition's small-launch
+    thresholds (at N = 2^14: 1 and 3 products share each chunk's prime tiles
+    over 4 waves, 5 over 2, 9 over 1), for each fused-stage count (split 0,
+    1, 2), give the oracle's bits."""
+    N = 16384
+    a = np.stack([ncmul.gen_uniform(700 + 2 * i, N, q868) for i in range(9)])
+    b = np.stack([ncmul.gen_uniform(701 + 2 * i, N, q868) for i in range(9)])
+    orc = O.Plan(N, q868)
+    want = [orc.multiply(a[i], b[i]) for i in range(9)]
+    d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+    d_c = ncmul.empty_device_u32((9, N, 28))
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_mfma(ncmul.MFMA_ALL)
+        for split in (0, 1, 2):
+            plan.set_tuning(30, split)
+            for B in (1, 3, 5, 9):
+                say("wave split", split, B)
+                d_c.zero_()
+                plan.multiply_device(d_a, d_b, d_c, B)
+                got = ncmul.from_device_u32(d_c)
+                for i in range(B):
+                    assert np.array_equal(got[i], want[i]), (split, B, i)
+
+
 @pytest.mark.parametrize("width,split", [(62, None), (30, 2), (30, 0)])
 def test_profile_stages(gpu, width, split, q868):
     """The stage profiler runs the real pipeline back to back: its last
