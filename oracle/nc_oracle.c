@@ -6,7 +6,7 @@
  * file; the arithmetic below restates the algorithm those lines implement,
  * written for a CPU with unsigned __int128.
  *
- * Parity pins (tests/test_oracle.py, tests/test_twiddles.py):
+ * Parity pins (tests/test_oracle.py):
  *   - orc_schoolbook is the definition itself; it is checked against the
  *     Python-int golden vectors (tests/golden) and closed forms.
  *   - the restated RNS path (orc_multiply, orc_crt_fast, orc_crt_garner) is
