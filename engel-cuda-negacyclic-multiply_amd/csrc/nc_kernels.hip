@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     const uint32_t col = lane & 31u, half = lane >> 5;
     const uint32_t nbytes = 4 * L;
     // persistent: the tables above are loaded once per workgroup and serve
-    // every work item it takes (work item = one polynomial's 4 NG chunks)
+    // every work item it takes (work item = 4 NG / WS chunks of one polynomial)
     for (uint32_t item = blockIdx.x; item < wgs_per_poly * (na + nb); item += gridDim.x) {
     const uint32_t poly = item / wgs_per_poly;
     const uint32_t chunk0 = (item - poly * wgs_per_poly) * kChunksPerItem;
