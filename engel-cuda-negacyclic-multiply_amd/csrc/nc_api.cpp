@@ -19,6 +19,8 @@ using nc::kMfmaAll;
 using nc::kMfmaCrt;
 using nc::kMfmaDecompose;
 
+static_assert(nc::kScratchBytes == NC_NOALLOC_SCRATCH_BYTES, "scratch size in the header and the library agree");
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -331,6 +333,10 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     };
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
+    // the allocation-free entry point's workspace: this device's scratch
+    // array, looked up once per device
+    e = nc::scratch_address(&p->scratch);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "scratch address"));
 
     // one resident allocation for both prime sets' tables
     const nc::HostSet62& a = t.s62;
@@ -721,6 +727,24 @@ int nc_multiply_device(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uin
     rc = grow_workspace(p, batch);
     if (rc) return rc;
     return run_pipeline(p, d_a, d_b, d_c, batch, static_cast<hipStream_t>(stream), choose(p, batch));
+}
+
+int nc_multiply_device_noalloc(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
+                               void* stream) {
+    if (!p || !d_a || !d_b || !d_c) return fail(NC_ERR_INVALID, "null argument");
+    if (int rc = check_batch(p, batch)) return rc;
+    if (!p->scratch) return fail(NC_ERR_STATE, "no scratch address for this plan's device");
+    if ((uint64_t)batch * 2 * p->res_bytes > nc::kScratchBytes)
+        return fail(NC_ERR_CAPACITY, "batch * bytes_per_product (" + std::to_string((uint64_t)batch * 2 * p->res_bytes) +
+                                         ") exceeds the scratch array (" + std::to_string(nc::kScratchBytes) + " bytes)");
+    // residue rows of a in the first half of what the batch uses, b after
+    // them: nothing is allocated, and nothing needs to start at zero (every
+    // residue is written before it is read)
+    char* ws = static_cast<char*>(p->scratch);
+    Choice c = choose(p, batch);
+    c.chunks = 1;
+    return nc::run_pipeline_ws(p, ws, ws + (size_t)batch * p->res_bytes, d_a, d_b, d_c, batch,
+                               static_cast<hipStream_t>(stream), c);
 }
 
 int nc_stage_residues(nc_plan* p, uint32_t width, const uint32_t* d_poly, void* d_res, uint32_t batch, void* stream) {

@@ -11,6 +11,10 @@ constexpr int kMaxLogN = 16;      // N <= 65536
 constexpr int kMaxLogBlock = 14;  // largest NTT block held in one workgroup's LDS
 constexpr int kMaxOuterBits = 4;  // outer NTT passes cover at most 4 stages
 
+// Bytes of the per-device scratch array of nc_multiply_device_noalloc
+// (residue rows of a and b): one product at N = 65536, four at N = 16384.
+constexpr unsigned long long kScratchBytes = 32ull << 20;
+
 // Per-prime constants for the NTT / pointwise kernels.
 struct PrimeConst {
     uint64_t p;         // prime, p < 2^62, p = 1 mod 2N

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "nc_arith.hpp"
@@ -1787,6 +1788,33 @@ hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* ou
                                                          d.L, d.w30.k, nprod);
     }
     return hipGetLastError();
+}
+
+// ===========================================================================
+// Scratch of nc_multiply_device_noalloc: the residue rows of a and b of an
+// allocation-free call.  Each device holds its own copy for the life of the
+// process; host code uses the address hipGetSymbolAddress returns for the
+// current device, never the array's name.
+// ===========================================================================
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t scratch_address(void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return hipSuccess;
 }
 
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s) {

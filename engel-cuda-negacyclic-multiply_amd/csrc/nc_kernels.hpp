@@ -107,6 +107,11 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 
+// Device address of the scratch array (kScratchBytes) on the current
+// device, looked up once per device and cached; the array's host-side name
+// is not a device address and is never passed to a kernel.
+hipError_t scratch_address(void** out);
+
 // Largest / smallest outer split the NTT kernels accept for this N.
 uint32_t max_log_blocks(uint32_t log_n);
 uint32_t min_log_blocks(uint32_t log_n);

@@ -94,6 +94,8 @@ struct nc_plan {
     void* d_res_b = nullptr;
     uint32_t ws_cap = 0;
     size_t res_bytes = 0;  // bytes of one product's residues (max over widths)
+    // device address of this device's scratch array (nc_multiply_device_noalloc)
+    void* scratch = nullptr;
     // staging for the host-pointer path (one product)
     uint32_t* d_io = nullptr;   // a | b | c, N*L u32 each
     uint32_t* h_pin = nullptr;  // pinned a | b | c

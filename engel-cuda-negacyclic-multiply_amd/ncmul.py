@@ -27,6 +27,8 @@ MFMA_DECOMPOSE = 1  # int8 matrix-core paths of the 30-bit prime set (nc_plan_se
 MFMA_CRT = 2
 MFMA_ALL = 3
 
+NOALLOC_SCRATCH_BYTES = 32 << 20  # NC_NOALLOC_SCRATCH_BYTES (negacyclic.h)
+
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
@@ -93,6 +95,7 @@ def lib():
         L.nc_plan_set_chunks.argtypes = [_vp, ctypes.c_uint32]
         L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
         L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
+        L.nc_multiply_device_noalloc.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_product_residues.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_stage_crt.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
@@ -282,6 +285,14 @@ class Plan:
     def multiply_device(self, d_a, d_b, d_c, batch: int, stream=None):
         _check(lib().nc_multiply_device(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()), _vp(d_c.data_ptr()),
                                         batch, _stream_handle(stream)), "nc_multiply_device")
+
+    def multiply_device_noalloc(self, d_a, d_b, d_c, batch: int, stream=None):
+        """nc_multiply_device with the library's per-device scratch array as
+        workspace (no device allocation; batch * bytes_per_product must fit
+        NOALLOC_SCRATCH_BYTES)."""
+        _check(lib().nc_multiply_device_noalloc(self._h, _vp(d_a.data_ptr()), _vp(d_b.data_ptr()),
+                                                _vp(d_c.data_ptr()), batch, _stream_handle(stream)),
+               "nc_multiply_device_noalloc")
 
     def stage_residues(self, width: int, d_poly, d_res, batch: int, stream=None):
         _check(lib().nc_stage_residues(self._h, width, _vp(d_poly.data_ptr()), _vp(d_res.data_ptr()), batch,

@@ -153,6 +153,24 @@ int nc_multiply(nc_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c
 int nc_multiply_device(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
                        void* stream);
 
+/* ---- allocation-free device batch ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, allocation-free serving loops,
+ * harnesses that forbid hipMalloc.  The same products as nc_multiply_device
+ * (bit-identical, same arguments, one chunk), with the residue workspace in
+ * a fixed device array of NC_NOALLOC_SCRATCH_BYTES per device that the
+ * library holds for the life of the process, instead of the plan's
+ * allocated workspace.  Calls no hipMalloc / hipFree and no allocating
+ * library routine.  batch * nc_plan_info.bytes_per_product must not exceed
+ * NC_NOALLOC_SCRATCH_BYTES (NC_ERR_CAPACITY otherwise): one product at
+ * N = 65536, four at N = 16384.  Every call on one device shares that
+ * device's array and nothing in it needs to start at zero; calls that
+ * overlap on one device (other streams, plans or threads) race on it, so
+ * the caller orders them. */
+#define NC_NOALLOC_SCRATCH_BYTES (32ull << 20)
+int nc_multiply_device_noalloc(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                               uint32_t batch, void* stream);
+
 /* ---- stage entry points (device pointers, enqueue on stream) ---------
  * For stage-level parity checks.  `width` picks the prime set; residues are
  * u64 for width 62 and u32 for width 30, layout [batch][k][N].
