@@ -168,6 +168,26 @@ int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t*
     return nc::run_pipeline_ws(p, p->d_res_a, p->d_res_b, d_a, d_b, d_c, batch, s, c);
 }
 
+// Orders every use of the plan's workspace across streams: the stream
+// waits for the previous user's work before it enqueues its own (a no-op on
+// the same stream), and the caller records ev_ws after enqueuing.  Under
+// stream capture neither happens: a captured graph orders its own nodes, and
+// its replays are ordered by the stream they are launched on.
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+int ws_acquire(nc_plan* p, hipStream_t s) {
+    if (!capturing(s)) NC_HIP(hipStreamWaitEvent(s, p->ev_ws, 0), "workspace order");
+    return NC_OK;
+}
+
+int ws_release(nc_plan* p, hipStream_t s) {
+    if (!capturing(s)) NC_HIP(hipEventRecord(p->ev_ws, s), "workspace order");
+    return NC_OK;
+}
+
 // Upload one host table vector at `off` of the table allocation.
 template <class V>
 hipError_t upload(char* base, size_t off, const V& v) {
@@ -199,10 +219,8 @@ int run_pipeline_ws(nc_plan* p, void* ws_a, void* ws_b, const uint32_t* d_a, con
     return NC_OK;
 }
 
-int ensure_workspace(nc_plan* p, uint32_t batch) {
-    if (int rc = check_batch(p, batch)) return rc;
-    return grow_workspace(p, batch);
-}
+int validate_batch(const nc_plan* p, uint32_t batch) { return check_batch(p, batch); }
+
 
 }  // namespace nc
 
@@ -243,6 +261,10 @@ void nc_plan_destroy(nc_plan* p) {
     for (hipEvent_t ev : p->ev_piece)
         if (ev) (void)hipEventDestroy(ev);
     if (p->ev_first) (void)hipEventDestroy(p->ev_first);
+    if (p->ev_ws) {
+        (void)hipEventSynchronize(p->ev_ws);
+        (void)hipEventDestroy(p->ev_ws);
+    }
     delete p;
 }
 
@@ -437,6 +459,7 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_first, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_ws, hipEventDisableTiming);
     for (int i = 0; i < nc_plan::kHostPieces && e == hipSuccess; ++i)
         e = hipEventCreateWithFlags(&p->ev_piece[i], hipEventDisableTiming);
     if (e != hipSuccess) return cleanup(hip_fail(e, "chunk streams"));
@@ -582,9 +605,14 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     rc = grow_workspace(p, batch);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (capturing(s)) return fail(NC_ERR_STATE, "autotune cannot run under stream capture");
     const size_t n = (size_t)batch * p->host.N * p->host.L;
     uint32_t* buf = nullptr;
     NC_HIP(hipMalloc(&buf, 3 * n * sizeof(uint32_t)), "autotune buffers");
+    if (int rc2 = ws_acquire(p, s)) {
+        (void)hipFree(buf);
+        return rc2;
+    }
     // zero inputs are valid (0 < q); the kernels' work does not depend on data
     hipError_t e = hipMemsetAsync(buf, 0, 2 * n * sizeof(uint32_t), s);
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -666,6 +694,7 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
         }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_ws, s);
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(e, "autotune");
@@ -704,7 +733,9 @@ int nc_multiply(nc_plan* p, const uint32_t* a, const uint32_t* b, uint32_t* c, s
             NC_HIP(hipMemcpyAsync(d + off, h + off, len, hipMemcpyHostToDevice, p->stream), "a/b copy");
         }
     }
-    int rc = run_pipeline(p, da, db, dc, 1, p->stream, choose(p, 1));
+    int rc = ws_acquire(p, p->stream);
+    if (rc == NC_OK) rc = run_pipeline(p, da, db, dc, 1, p->stream, choose(p, 1));
+    if (rc == NC_OK) rc = ws_release(p, p->stream);
     if (rc) return rc;
     for (int i = 0; i < pieces; ++i) {
         const size_t off = i * pw, len = std::min(pw, n - off) * sizeof(uint32_t);
@@ -726,7 +757,11 @@ int nc_multiply_device(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uin
     if (rc) return rc;
     rc = grow_workspace(p, batch);
     if (rc) return rc;
-    return run_pipeline(p, d_a, d_b, d_c, batch, static_cast<hipStream_t>(stream), choose(p, batch));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    rc = ws_acquire(p, s);
+    if (rc == NC_OK) rc = run_pipeline(p, d_a, d_b, d_c, batch, s, choose(p, batch));
+    if (rc == NC_OK) rc = ws_release(p, s);
+    return rc;
 }
 
 int nc_multiply_device_noalloc(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch,
@@ -770,11 +805,12 @@ int nc_stage_product_residues(nc_plan* p, uint32_t width, const uint32_t* d_a, c
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool dm = (c.mfma & kMfmaDecompose) != 0;
     const bool fuse = nc::decompose_fuses_outer(p->dev, (int)width, dm, c.log_blocks);
+    if (int rc2 = ws_acquire(p, s)) return rc2;
     NC_HIP(nc::launch_decompose(p->dev, (int)width, d_a, d_res, batch, d_b, p->d_res_b, batch, s, dm,
                                 fuse ? c.log_blocks : 0u),
            "decompose launch");
     NC_HIP(nc::launch_ntt_product(p->dev, (int)width, d_res, p->d_res_b, batch, c.log_blocks, s, fuse), "ntt launch");
-    return NC_OK;
+    return ws_release(p, s);
 }
 
 int nc_stage_crt(nc_plan* p, uint32_t width, const void* d_res, uint32_t* d_c, uint32_t batch, void* stream) {
@@ -810,6 +846,7 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
     }
     double acc[5] = {0, 0, 0, 0, 0};
     const bool inv = inv_in_crt(p, c);  // then the CRT time includes the last inverse stages
+    if (status == NC_OK) status = ws_acquire(p, s);
     hipError_t e = hipSuccess;
     for (uint32_t r = 0; r < reps && status == NC_OK && e == hipSuccess; ++r) {
         hipEvent_t* er = &ev[6 * (size_t)r];
@@ -830,6 +867,7 @@ int nc_profile_stages(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint
                     : nc::launch_crt(p->dev, w, p->d_res_a, d_c, batch, s, (c.mfma & kMfmaCrt) != 0);
         if (e == hipSuccess) e = hipEventRecord(er[5], s);
     }
+    if (status == NC_OK && e == hipSuccess) status = ws_release(p, s);
     if (status == NC_OK && e == hipSuccess) e = hipStreamSynchronize(s);
     for (uint32_t r = 0; r < reps && status == NC_OK && e == hipSuccess; ++r)
         for (int i = 0; i < 5 && e == hipSuccess; ++i) {

@@ -131,7 +131,7 @@ int nc_batch_capture(nc_batch* b, const nc_group* groups, uint32_t n, nc_graph**
     for (uint32_t i = 0; i < n; ++i) {
         nc_plan* p = nullptr;
         if (int rc = nc_batch_plan(b, groups[i].N, &p)) return rc;
-        if (int rc = nc::ensure_workspace(p, groups[i].count)) return rc;  // validates the count
+        if (int rc = nc::validate_batch(p, groups[i].count)) return rc;  // the graph owns its workspace
         if (!by_degree.count(groups[i].N)) order.push_back(groups[i].N);
         by_degree[groups[i].N].push_back(i);
     }

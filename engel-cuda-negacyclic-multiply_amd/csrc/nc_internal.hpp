@@ -27,8 +27,9 @@ size_t workspace_bytes_per_product(const nc_plan* p);
 int run_pipeline_ws(nc_plan* p, void* ws_a, void* ws_b, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
                     uint32_t batch, hipStream_t s, const Choice& c);
 
-// Validates `batch` and grows the plan's own workspace to it (may
-// synchronise the device; never call under capture).
-int ensure_workspace(nc_plan* p, uint32_t batch);
+// Validates `batch` for this plan (count and N limits) without touching
+// the plan's workspace.
+int validate_batch(const nc_plan* p, uint32_t batch);
+
 
 }  // namespace nc

@@ -104,6 +104,10 @@ struct nc_plan {
     // that fork them from, stagger them and join them back to the caller's stream
     hipStream_t sub[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_first = nullptr, ev_join[2] = {nullptr, nullptr};
+    // the last enqueued work that uses d_res_a / d_res_b: every call that
+    // uses them first makes its stream wait for it, then records it, so
+    // calls on different streams reach the workspace one after another
+    hipEvent_t ev_ws = nullptr;
     // host-path staging pipeline: one event per output piece
     static constexpr int kHostPieces = 4;
     hipEvent_t ev_piece[kHostPieces] = {};

@@ -3,22 +3,27 @@
 //
 // One scheduler thread owns the device work.  Per degree it keeps a queue of
 // waiting requests and `lanes_per_degree` lanes; a lane is one in-flight
-// batch with its own stream, event, residue workspace and device staging for
-// max_batch products.  Each turn the scheduler (1) moves newly submitted
-// requests to their degree's queue, (2) retires lanes whose event has fired
-// (requests complete, their device bytes are released), and (3) fills every
-// idle lane with waiting requests of its degree, as many as max_batch and the
-// admission budget allow: H2D copies of the operands, the batched pipeline,
-// D2H copies of the results, then the lane's event, all on the lane's
-// stream, so copies of one batch overlap kernels of others.  Requests that
-// arrive while lanes are busy are admitted as soon as a lane and budget free
-// up (the next turn), not after the queue drains.
+// batch with its own stream and event.  Every batch's device memory
+// (operands, results, residue workspace) is one block of a single device
+// arena of at most hbm_budget bytes, allocated at creation: a request is
+// admitted only when its batch's block fits in the arena, so admitted
+// requests never hold more device memory than the budget.  Each turn the
+// scheduler (1) moves newly submitted requests to their degree's queue,
+// (2) retires lanes whose event has fired (requests complete, their block
+// returns to the arena), and (3) fills every idle lane with waiting requests
+// of its degree, as many as max_batch and the arena allow: H2D copies of the
+// operands, the batched pipeline, D2H copies of the results, then the lane's
+// event, all on the lane's stream, so copies of one batch overlap kernels of
+// others.  Requests that arrive while lanes are busy are admitted as soon as
+// a lane and arena space free up (the next turn), not after the queue
+// drains.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <new>
@@ -37,6 +42,8 @@ using Clock = std::chrono::steady_clock;
 
 double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
 
+uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
+
 struct Req {
     uint64_t id = 0;
     uint32_t N = 0;
@@ -49,24 +56,83 @@ struct Req {
     int status = NC_OK;
 };
 
+// The device memory of admitted batches: one allocation, handed out as
+// first-fit blocks; a freed block merges with free neighbours.  Used by the
+// scheduler thread only.
+struct Arena {
+    char* base = nullptr;
+    uint64_t size = 0, used = 0;
+    std::map<uint64_t, uint64_t> free_blocks;  // offset -> bytes
+
+    void reset(char* b, uint64_t n) {
+        base = b;
+        size = n;
+        used = 0;
+        free_blocks.clear();
+        if (n) free_blocks[0] = n;
+    }
+    bool alloc(uint64_t bytes, uint64_t* off) {
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - bytes, at = it->first + bytes;
+            free_blocks.erase(it);
+            if (rest) free_blocks[at] = rest;
+            used += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        if (!bytes) return;
+        used -= bytes;
+        auto next = free_blocks.lower_bound(off);
+        if (next != free_blocks.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_blocks.erase(next);
+        }
+        if (next != free_blocks.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_blocks[off] = bytes;
+    }
+};
+
 struct Lane {
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr;
-    void* ws_a = nullptr;
-    void* ws_b = nullptr;
-    uint32_t* io = nullptr;  // a | b | c, cap products each
     std::vector<Req*> batch;
-    uint64_t bytes = 0;
+    uint64_t off = 0, bytes = 0;  // the batch's arena block
     bool busy = false;
 };
 
 struct Degree {
     uint32_t N = 0;
     nc_plan* plan = nullptr;
-    uint64_t footprint = 0;
+    uint64_t words = 0;      // u32 limbs of one operand
+    uint64_t ws = 0;         // residue bytes of one operand
+    uint64_t footprint = 0;  // block bytes of one request
     uint32_t cap = 0;
     std::deque<Req*> queue;
     std::vector<Lane> lanes;
+
+    // Block of n products: a | b | c (n N L u32 each, product-major), then
+    // the residue rows of a and of b (n ws bytes each), every part 256-byte
+    // aligned; block_bytes(n) <= n block_bytes(1).
+    uint64_t block_bytes(uint32_t n) const { return 3 * align256(n * words * 4) + 2 * align256(n * ws); }
+    struct Parts {
+        uint32_t *a, *b, *c;
+        void *ws_a, *ws_b;
+    };
+    Parts parts(char* block, uint32_t n) const {
+        const uint64_t io = align256(n * words * 4), w = align256(n * ws);
+        return Parts{reinterpret_cast<uint32_t*>(block), reinterpret_cast<uint32_t*>(block + io),
+                     reinterpret_cast<uint32_t*>(block + 2 * io), block + 3 * io, block + 3 * io + w};
+    }
 };
 
 }  // namespace
@@ -85,7 +151,8 @@ struct nc_server {
     bool ready = false;                       // under mu: lanes warmed (or failed)
     int warm_rc = NC_OK;                      // under mu
     nc_server_stats st{};                     // under mu
-    uint64_t inflight = 0;                    // scheduler thread only
+    Arena arena;                              // scheduler thread only (after creation)
+    void* arena_mem = nullptr;
     std::thread th;
 };
 
@@ -105,36 +172,36 @@ void finish(nc_server* sv, Lane& ln, int status) {
         sv->st.completed += ln.batch.size();
     }
     sv->cv_done.notify_all();
-    sv->inflight -= ln.bytes;
+    sv->arena.release(ln.off, ln.bytes);
     ln.bytes = 0;
     ln.batch.clear();
     ln.busy = false;
 }
 
-// Enqueues one batch on an idle lane; returns false if nothing was admitted.
+// Enqueues one batch on an idle lane: the largest batch (up to max_batch)
+// whose block fits in the arena now.  Returns false if none fits.
 bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
-    const uint64_t budget = sv->cfg.hbm_budget;
-    while (!d.queue.empty() && ln.batch.size() < d.cap && sv->inflight + d.footprint <= budget) {
+    uint32_t n = (uint32_t)std::min<size_t>(d.queue.size(), d.cap);
+    for (; n > 0; --n)
+        if (sv->arena.alloc(d.block_bytes(n), &ln.off)) break;
+    if (n == 0) return false;
+    ln.bytes = d.block_bytes(n);
+    for (uint32_t i = 0; i < n; ++i) {
         ln.batch.push_back(d.queue.front());
         d.queue.pop_front();
-        sv->inflight += d.footprint;
-        ln.bytes += d.footprint;
     }
-    if (ln.batch.empty()) return false;
-    const uint32_t n = (uint32_t)ln.batch.size();
-    const size_t words = (size_t)d.N * sv->cfg.L;
-    uint32_t* da = ln.io;
-    uint32_t* db = ln.io + (size_t)d.cap * words;
-    uint32_t* dc = ln.io + 2 * (size_t)d.cap * words;
+    const size_t words = d.words;
+    const Degree::Parts pt = d.parts(sv->arena.base + ln.off, n);
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
-        e = hipMemcpyAsync(da + i * words, ln.batch[i]->a, words * 4, hipMemcpyHostToDevice, ln.s);
-        if (e == hipSuccess) e = hipMemcpyAsync(db + i * words, ln.batch[i]->b, words * 4, hipMemcpyHostToDevice, ln.s);
+        e = hipMemcpyAsync(pt.a + i * words, ln.batch[i]->a, words * 4, hipMemcpyHostToDevice, ln.s);
+        if (e == hipSuccess) e = hipMemcpyAsync(pt.b + i * words, ln.batch[i]->b, words * 4, hipMemcpyHostToDevice, ln.s);
     }
     int rc = e == hipSuccess ? NC_OK : hip_err(e, "request copy in");
-    if (rc == NC_OK) rc = nc::run_pipeline_ws(d.plan, ln.ws_a, ln.ws_b, da, db, dc, n, ln.s, nc::choose_for(d.plan, n));
+    if (rc == NC_OK)
+        rc = nc::run_pipeline_ws(d.plan, pt.ws_a, pt.ws_b, pt.a, pt.b, pt.c, n, ln.s, nc::choose_for(d.plan, n));
     for (uint32_t i = 0; i < n && rc == NC_OK; ++i) {
-        e = hipMemcpyAsync(ln.batch[i]->c, dc + i * words, words * 4, hipMemcpyDeviceToHost, ln.s);
+        e = hipMemcpyAsync(ln.batch[i]->c, pt.c + i * words, words * 4, hipMemcpyDeviceToHost, ln.s);
         if (e != hipSuccess) rc = hip_err(e, "request copy out");
     }
     if (rc == NC_OK) {
@@ -145,7 +212,7 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
         std::lock_guard<std::mutex> g(sv->mu);
         sv->st.launches += 1;
         sv->st.max_batch_seen = std::max<uint64_t>(sv->st.max_batch_seen, n);
-        sv->st.peak_bytes = std::max<uint64_t>(sv->st.peak_bytes, sv->inflight);
+        sv->st.peak_bytes = std::max<uint64_t>(sv->st.peak_bytes, sv->arena.used);
     }
     if (rc != NC_OK) {
         (void)hipStreamSynchronize(ln.s);
@@ -158,12 +225,13 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
 
 // Warms every lane before the first request, on the scheduler thread (the
 // thread that issues all lane work later): copies in from pinned memory,
-// the pipeline at every batch-size bucket (1, 2, 4, .., cap: each bucket's
-// tuning may use kernels nothing has launched yet, and a kernel's first
-// launch in a process is slow), the copy out, all on the lane's own stream.
-// Measured at 6000 requests/s: without it the first ~0.6 s of a process ran
-// behind (p99 ~40 ms, against ~4 ms later); with it the first ~0.1 s still
-// does.
+// the pipeline at every batch-size bucket (1, 2, 4, .., max_batch, or the
+// largest whose block fits the arena: each bucket's tuning may use kernels
+// nothing has launched yet, and a kernel's first launch in a process is
+// slow), the copy out, all on the lane's own stream, in arena blocks that
+// are returned afterwards.  Measured at 6000 requests/s: without it the
+// first ~0.6 s of a process ran behind (p99 ~40 ms, against ~4 ms later);
+// with it the first ~0.1 s still does.
 int warm_lanes(nc_server* sv) {
     const uint32_t L = sv->cfg.L;
     size_t max_words = 0;
@@ -175,24 +243,34 @@ int warm_lanes(nc_server* sv) {
     int rc = NC_OK;
     for (auto& kv : sv->deg) {
         Degree& d = kv.second;
-        const size_t words = (size_t)d.N * L;
+        const size_t words = d.words;
+        uint32_t top = d.cap;  // the arena is all free here
+        while (top > 1 && d.block_bytes(top) > sv->arena.size) --top;
         for (Lane& ln : d.lanes) {
-            uint32_t* da = ln.io;
-            uint32_t* db = ln.io + (size_t)d.cap * words;
-            uint32_t* dc = ln.io + 2 * (size_t)d.cap * words;
-            e = hipMemsetAsync(ln.io, 0, 3 * (size_t)d.cap * words * 4, ln.s);
-            if (e == hipSuccess) e = hipMemcpyAsync(da, h, words * 4, hipMemcpyHostToDevice, ln.s);
-            if (e == hipSuccess) e = hipMemcpyAsync(db, h + max_words, words * 4, hipMemcpyHostToDevice, ln.s);
-            if (e != hipSuccess) rc = hip_err(e, "warm-up copy in");
-            for (uint32_t n = 1; rc == NC_OK; n = std::min(2 * n, d.cap)) {
-                rc = nc::run_pipeline_ws(d.plan, ln.ws_a, ln.ws_b, da, db, dc, n, ln.s, nc::choose_for(d.plan, n));
-                if (n == d.cap) break;
+            uint64_t off = 0;
+            if (!sv->arena.alloc(d.block_bytes(top), &off)) {
+                rc = set_error(NC_ERR_CAPACITY, "request arena below one request");
+                break;
             }
-            if (rc == NC_OK) {
-                e = hipMemcpyAsync(h + 2 * max_words, dc, words * 4, hipMemcpyDeviceToHost, ln.s);
-                if (e == hipSuccess) e = hipStreamSynchronize(ln.s);
-                if (e != hipSuccess) rc = hip_err(e, "warm-up");
+            char* block = sv->arena.base + off;
+            e = hipMemsetAsync(block, 0, d.block_bytes(top), ln.s);  // zero operands are valid (0 < q)
+            for (uint32_t n = 1; rc == NC_OK; n = std::min(2 * n, top)) {
+                const Degree::Parts pt = d.parts(block, n);
+                if (e == hipSuccess) e = hipMemcpyAsync(pt.a, h, words * 4, hipMemcpyHostToDevice, ln.s);
+                if (e == hipSuccess) e = hipMemcpyAsync(pt.b, h + max_words, words * 4, hipMemcpyHostToDevice, ln.s);
+                if (e != hipSuccess) rc = hip_err(e, "warm-up copy in");
+                if (rc == NC_OK)
+                    rc = nc::run_pipeline_ws(d.plan, pt.ws_a, pt.ws_b, pt.a, pt.b, pt.c, n, ln.s,
+                                             nc::choose_for(d.plan, n));
+                if (rc == NC_OK) {
+                    e = hipMemcpyAsync(h + 2 * max_words, pt.c, words * 4, hipMemcpyDeviceToHost, ln.s);
+                    if (e != hipSuccess) rc = hip_err(e, "warm-up copy out");
+                }
+                if (n == top) break;
             }
+            const hipError_t es = hipStreamSynchronize(ln.s);  // before the block returns to the arena
+            if (rc == NC_OK && es != hipSuccess) rc = hip_err(es, "warm-up");
+            sv->arena.release(off, d.block_bytes(top));
             if (rc != NC_OK) break;
         }
         if (rc != NC_OK) break;
@@ -268,9 +346,6 @@ void free_degree(Degree& d) {
         if (ln.s) (void)hipStreamSynchronize(ln.s);
         if (ln.ev) (void)hipEventDestroy(ln.ev);
         if (ln.s) (void)hipStreamDestroy(ln.s);
-        if (ln.ws_a) (void)hipFree(ln.ws_a);
-        if (ln.ws_b) (void)hipFree(ln.ws_b);
-        if (ln.io) (void)hipFree(ln.io);
     }
     if (d.plan) nc_plan_destroy(d.plan);
 }
@@ -298,7 +373,8 @@ void nc_server_destroy(nc_server* sv) {
         sv->cv_work.notify_all();
         sv->th.join();
     }
-    for (auto& kv : sv->deg) free_degree(kv.second);
+    for (auto& kv : sv->deg) free_degree(kv.second);  // every lane stream is drained first
+    if (sv->arena_mem) (void)hipFree(sv->arena_mem);
     for (auto& kv : sv->reqs) delete kv.second;
     delete sv;
 }
@@ -309,6 +385,7 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
     if (!cfg || !cfg->q || !cfg->degrees || cfg->n_degrees == 0) return set_error(NC_ERR_INVALID, "bad config");
     if (cfg->L == 0 || cfg->L > 32) return set_error(NC_ERR_INVALID, "L must be in [1, 32]");
     if (cfg->max_batch == 0 || cfg->lanes_per_degree == 0) return set_error(NC_ERR_INVALID, "max_batch and lanes must be >= 1");
+    if (cfg->lanes_per_degree > 64) return set_error(NC_ERR_INVALID, "lanes_per_degree above 64");
     nc_server* sv = new (std::nothrow) nc_server();
     if (!sv) return set_error(NC_ERR_NOMEM, "server");
     sv->cfg = *cfg;
@@ -321,19 +398,21 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
     };
     hipError_t e = hipGetDevice(&sv->device);
     if (e != hipSuccess) return fail_out(hip_err(e, "hipGetDevice"));
+    uint64_t most = 0;  // every lane of every degree holding a full batch at once
     for (uint32_t i = 0; i < cfg->n_degrees; ++i) {
         const uint32_t N = cfg->degrees[i];
         if (sv->deg.count(N)) continue;
+        if ((uint64_t)cfg->max_batch * N > (1ull << 31)) return fail_out(set_error(NC_ERR_CAPACITY, "max_batch * N above 2^31"));
         Degree& d = sv->deg[N];
         d.N = N;
         if (int rc = nc_plan_create(N, cfg->L, sv->q.data(), &d.plan)) return fail_out(rc);
-        const size_t words = (size_t)N * cfg->L;
-        const size_t ws = nc::workspace_bytes_per_product(d.plan);
-        d.footprint = 3 * words * 4 + 2 * ws;
+        d.words = (uint64_t)N * cfg->L;
+        d.ws = nc::workspace_bytes_per_product(d.plan);
+        d.footprint = d.block_bytes(1);
         if (d.footprint > cfg->hbm_budget)
             return fail_out(set_error(NC_ERR_CAPACITY, "hbm_budget below one request of degree " + std::to_string(N)));
-        if ((uint64_t)cfg->max_batch * N > (1ull << 31)) return fail_out(set_error(NC_ERR_CAPACITY, "max_batch * N above 2^31"));
         d.cap = cfg->max_batch;
+        most += (uint64_t)cfg->lanes_per_degree * d.block_bytes(d.cap);
         if (cfg->autotune) {
             for (uint32_t bsz : {1u, d.cap})
                 if (int rc = nc_plan_autotune(d.plan, bsz, nullptr, nullptr, nullptr)) return fail_out(rc);
@@ -342,13 +421,17 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
         for (Lane& ln : d.lanes) {
             e = hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
-            if (e == hipSuccess) e = hipMalloc(&ln.ws_a, (size_t)d.cap * ws);
-            if (e == hipSuccess) e = hipMalloc(&ln.ws_b, (size_t)d.cap * ws);
-            if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ln.io), 3 * (size_t)d.cap * words * 4);
-            if (e != hipSuccess) return fail_out(hip_err(e, "lane allocation"));
-            sv->st.device_bytes += 2 * (size_t)d.cap * ws + 3 * (size_t)d.cap * words * 4;
+            if (e != hipSuccess) return fail_out(hip_err(e, "lane streams"));
         }
     }
+    // the arena: hbm_budget bytes, or what every lane holding a full batch
+    // at once needs, if that is less
+    const uint64_t arena = std::min<uint64_t>(cfg->hbm_budget & ~255ull, most);
+    e = hipMalloc(&sv->arena_mem, arena);
+    if (e != hipSuccess)
+        return fail_out(set_error(NC_ERR_NOMEM, "request arena of " + std::to_string(arena) + " bytes: " + hipGetErrorString(e)));
+    sv->arena.reset(static_cast<char*>(sv->arena_mem), arena);
+    sv->st.device_bytes = arena;
     sv->th = std::thread(scheduler, sv);
     {  // the scheduler thread warms its lanes before taking requests; creation reports how that went
         std::unique_lock<std::mutex> lk(sv->mu);

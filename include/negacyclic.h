@@ -22,8 +22,15 @@
  * Every entry point returns NC_OK (0) or a negative status; nc_last_error()
  * gives the detail of the last failure on the calling thread.
  *
- * Threading: a plan is not re-entrant (like the reference's State,
- * solve.cu:28-65); use one plan per host thread, or serialise.
+ * Threading: a plan's host state is not re-entrant (like the reference's
+ * State, solve.cu:28-65): calls on one plan from several host threads must
+ * be serialised.  Its device work may go to any streams: every call that
+ * uses the plan's residue workspace (nc_multiply, nc_multiply_device,
+ * nc_stage_product_residues, nc_profile_stages, nc_plan_autotune) first
+ * makes its stream wait for the previous such call's work, so calls on
+ * different streams reach the workspace one after another, as the
+ * reference's calls do on its one stream (solve.cu:1489-1521).  Under
+ * stream capture the captured graph carries no such wait.
  */
 #ifndef NEGACYCLIC_H
 #define NEGACYCLIC_H

@@ -58,10 +58,15 @@ void nc_graph_destroy(nc_graph* graph);
  * keeps up to `lanes_per_degree` batches of each degree in flight on their
  * own streams (continuous batching: requests that arrive while others run
  * join the next batch as lanes free up); each batch's copies in and out
- * overlap other batches' kernels.  A request is admitted only while the
- * device bytes of admitted, unfinished requests (its footprint: operands,
- * result and residue workspace, nc_server_footprint) stay within
- * hbm_budget.  Host buffers from nc_host_alloc (pinned) let copies overlap;
+ * overlap other batches' kernels.  The device memory of requests is one
+ * arena the server allocates at creation: hbm_budget bytes, or less when
+ * every lane of every degree holding max_batch requests at once needs less
+ * (nc_server_stats.device_bytes reports it).  Each batch's operands, results
+ * and residue workspace are one block of that arena (at most
+ * count x nc_server_footprint bytes), and a waiting request is admitted only
+ * when its batch's block fits, so the device bytes of admitted, unfinished
+ * requests never exceed hbm_budget; plan tables (nc_plan_info.table_bytes
+ * per degree) are outside it.  Host buffers from nc_host_alloc (pinned) let copies overlap;
  * other host memory works but copies synchronously. */
 typedef struct nc_server_config {
     uint32_t L;
@@ -70,7 +75,7 @@ typedef struct nc_server_config {
     uint32_t n_degrees;
     uint32_t max_batch;        /* products per launch (per lane)            */
     uint32_t lanes_per_degree; /* batches of one degree in flight at once   */
-    uint64_t hbm_budget;       /* device bytes of admitted requests          */
+    uint64_t hbm_budget;       /* bound on the request arena (device bytes)  */
     uint32_t autotune;         /* 1: autotune each degree at creation         */
 } nc_server_config;
 
@@ -79,9 +84,9 @@ typedef struct nc_server_stats {
     uint64_t completed;
     uint64_t launches;           /* batches run                              */
     uint64_t max_batch_seen;     /* largest batch launched                   */
-    uint64_t peak_bytes;         /* most device bytes admitted at once       */
+    uint64_t peak_bytes;         /* most arena bytes in use at once          */
     uint64_t admission_waits;    /* times a waiting request did not fit      */
-    uint64_t device_bytes;       /* device memory the server holds           */
+    uint64_t device_bytes;       /* the request arena's size                 */
 } nc_server_stats;
 
 typedef struct nc_server nc_server;

@@ -174,7 +174,9 @@ def test_server_admission_budget(gpu, q868):
             srv.wait(t, timeout_ms=60000)
         st = srv.stats()
     say("admission stats", st, "budget", budget)
-    assert st["peak_bytes"] <= budget and st["admission_waits"] > 0
+    # the arena (all the device memory requests get) stays within the budget
+    assert st["device_bytes"] <= budget
+    assert st["peak_bytes"] <= st["device_bytes"] and st["admission_waits"] > 0
     for i, (N, a, b, c) in enumerate(reqs):
         assert np.array_equal(c, alone[i]), (i, N)
 
@@ -202,6 +204,31 @@ def test_server_poisson_replay(gpu, q868):
         st = srv.stats()
     say("replay", n, "makespan", makespan, "p50", np.percentile(lat, 50), "p99", np.percentile(lat, 99), st)
     assert st["completed"] == n and (lat > 0).all() and makespan >= arrivals[-1]
-    alone = _alone(q868, [reqs[i] for i in range(0, n, 10)])
-    for j, i in enumerate(range(0, n, 10)):
-        assert np.array_equal(reqs[i][3], alone[j]), i
+    # every response, each with its own operands and result buffer
+    alone = _alone(q868, reqs)
+    for i in range(n):
+        assert np.array_equal(reqs[i][3], alone[i]), i
+
+
+def test_server_arena_bounds_device_memory(gpu, q868):
+    """The request arena is min(hbm_budget, every lane full) bytes; with a
+    budget below what max_batch x lanes would take, batches shrink to what
+    fits and every result stays exact."""
+    degrees = [16384] * 12
+    reqs = _requests(degrees, q868, 11000)
+    alone = _alone(q868, reqs)
+    with ncmul.Server(q868, [16384], max_batch=8, lanes=2, hbm_budget=1 << 40, autotune=False) as probe:
+        f = probe.footprint(16384)
+        full = probe.stats()["device_bytes"]
+    assert full <= 2 * 8 * f  # no more than every lane holding a full batch
+    budget = 3 * f + 1000  # three requests at a time, not 16
+    with ncmul.Server(q868, [16384], max_batch=8, lanes=2, hbm_budget=budget, autotune=False) as srv:
+        tickets = [srv.submit(a, b, c) for _, a, b, c in reqs]
+        for t in tickets:
+            srv.wait(t, timeout_ms=60000)
+        st = srv.stats()
+    say("arena stats", st, "budget", budget, "footprint", f)
+    assert st["device_bytes"] <= budget and st["peak_bytes"] <= budget
+    assert st["max_batch_seen"] <= 3  # max_batch is 8: the arena, not the lanes, limited the batches
+    for i in range(len(reqs)):
+        assert np.array_equal(reqs[i][3], alone[i]), i

@@ -295,9 +295,57 @@ def test_autotune_keeps_bits(gpu, q868):
     with ncmul.Plan(N, q868) as plan:
         for batch in (1, 8):
             w, lb = plan.autotune(batch)
-            say("autotune", batch, w, lb)
+            say("autotune", batch, w, lb, "choice", plan.choice(batch))
             assert w in WIDTHS
         assert np.array_equal(plan.multiply(a, b), want)
+        # the batch-8 choice (possibly chunked) through the device entry
+        # point, every product against the oracle
+        A = np.stack([ncmul.gen_uniform(63 + 2 * i, N, q868) for i in range(8)])
+        B = np.stack([ncmul.gen_uniform(64 + 2 * i, N, q868) for i in range(8)])
+        d_c = ncmul.empty_device_u32((8, N, 28))
+        d_c.fill_(-1)
+        plan.multiply_device(ncmul.to_device_u32(A), ncmul.to_device_u32(B), d_c, 8)
+        got = ncmul.from_device_u32(d_c)
+        oplan = O.Plan(N, q868)
+        for i in range(8):
+            assert np.array_equal(got[i], oplan.multiply(A[i], B[i])), i
+
+
+def test_one_plan_two_streams(gpu, q868):
+    """Calls on one plan from different streams share its residue
+    workspace; the plan orders them on the device.  A long batch on a side
+    stream, then the host-path call (the plan's own stream) and a device call
+    on a second side stream, with no synchronisation in between: every
+    product equals the oracle's."""
+    import torch
+
+    N, n = 4096, 48
+    A = np.stack([ncmul.gen_uniform(700 + 2 * i, N, q868) for i in range(n)])
+    B = np.stack([ncmul.gen_uniform(701 + 2 * i, N, q868) for i in range(n)])
+    oplan = O.Plan(N, q868)
+    want = [oplan.multiply(A[i], B[i]) for i in range(n)]
+    d_a, d_b = ncmul.to_device_u32(A), ncmul.to_device_u32(B)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_chunks(1)
+        for rep in range(3):
+            c1 = ncmul.empty_device_u32((n, N, 28))
+            c2 = ncmul.empty_device_u32((4, N, 28))
+            c1.fill_(-1)
+            c2.fill_(-1)
+            torch.cuda.synchronize()
+            plan.multiply_device(d_a, d_b, c1, n, stream=s1)      # long, asynchronous
+            host = plan.multiply(A[5], B[5])                        # plan stream, blocks
+            plan.multiply_device(d_a[8:12], d_b[8:12], c2, 4, stream=s2)
+            plan.multiply_device(d_b, d_a, c1, n, stream=s1)        # commuted: same products
+            torch.cuda.synchronize()
+            say("two streams rep", rep)
+            assert np.array_equal(host, want[5])
+            got1, got2 = ncmul.from_device_u32(c1), ncmul.from_device_u32(c2)
+            for i in range(n):
+                assert np.array_equal(got1[i], want[i]), (rep, i)
+            for i in range(4):
+                assert np.array_equal(got2[i], want[8 + i]), (rep, i)
 
 
 @pytest.mark.parametrize("N", [2, 1024, 16384, 65536])
