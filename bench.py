@@ -74,6 +74,35 @@ def bytes_per_stage(N: int, L: int, k: int, word_bytes: int, batch: int) -> dict
     }
 
 
+INT8_PEAK = 5.0e15  # ops/s, MI355X dense int8 matrix-core peak (published)
+
+
+def int8_ops_per_stage(N: int, k30: int, batch: int) -> dict:
+    """int8 matrix-core operations (multiply and add counted apart) the two
+    v_mfma_i32_32x32x32_i8 stages issue per launch for L = 28, 30-bit primes:
+    decomposition: per polynomial, (N / 32) coefficient tiles x ceil(k / 8)
+    prime tiles x 4 MFMAs (128 byte columns); CRT: per 64 coefficients,
+    2 tiles x 4 row tiles x ceil(k / 8) chunks MFMAs.  One MFMA is
+    32 x 32 x 32 multiply-adds."""
+    mfma_ops = 2 * 32 * 32 * 32
+    tiles = (k30 + 7) // 8
+    return {
+        "decompose": 2 * batch * (N // 32) * tiles * 4 * mfma_ops,
+        "crt": batch * (N // 64) * 2 * 4 * tiles * mfma_ops,
+    }
+
+
+def trace_avg_us(kernel: str, N: int, width: int, split: int, batch: int):
+    """Average duration per launch (us) of `kernel` in the committed
+    rocprofv3 kernel-trace summary of this configuration
+    (profiles/kernel_trace_N{N}_w{width}_s{split}_b{batch}.md), or None."""
+    f = ROOT / "profiles" / f"kernel_trace_N{N}_w{width}_s{split}_b{batch}.md"
+    if not f.exists():
+        return None, None
+    m = re.search(rf"^trace_avg_us\s+{kernel}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+    return (float(m.group(1)), f.name) if m else (None, None)
+
+
 def pmc_traffic(kernel: str, N: int, width: int, split: int, batch: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary for this exact configuration
@@ -145,66 +174,169 @@ def mixed_batch(q: np.ndarray, n: int, seed: int, launches: int):
     }
 
 
-def served(q: np.ndarray, n: int, rate: float, seed: int):
-    """n requests of degree log-uniform in {2^11, ..., 2^16}, Poisson arrivals
-    at `rate`, replayed open loop into the continuous-batching server
-    (host operands in pinned memory, copies included); against the plain
-    form on the same degree mix: one host-to-host nc_multiply at a time."""
+SERVE_DEGREES = [1 << e for e in range(11, 17)]
+
+
+class ServeMix:
+    """Requests of degree log-uniform in {2^11, ..., 2^16}: `sets` operand
+    pairs per degree in pinned memory (request j of a degree uses set
+    j % sets) and their products from the plain host path (nc_multiply, the
+    oracle-pinned fherma_run replacement), against which every response is
+    checked."""
+
+    def __init__(self, q: np.ndarray, sets: int):
+        import ncmul
+
+        self.q, self.L, self.sets = q, q.shape[0], sets
+        self.ops, self.want = {}, {}
+        for N in SERVE_DEGREES:
+            a = ncmul.host_array((sets, N, self.L))
+            b = ncmul.host_array((sets, N, self.L))
+            for j in range(sets):
+                a[j] = ncmul.gen_uniform(2 * (20000 + 100 * j + N.bit_length()), N, q)
+                b[j] = ncmul.gen_uniform(2 * (20000 + 100 * j + N.bit_length()) + 1, N, q)
+            self.ops[N] = (a, b)
+            with ncmul.Plan(N, q) as plan:
+                self.want[N] = np.stack([plan.multiply(a[j], b[j]) for j in range(sets)])
+
+    def stream(self, n: int, rate: float, seed: int):
+        """(degrees, arrival times, set index per request) of an open-loop Poisson stream."""
+        rng = np.random.default_rng(seed)
+        degrees = [1 << int(x) for x in rng.integers(11, 17, size=n)]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        cnt = {N: 0 for N in SERVE_DEGREES}
+        sets = []
+        for N in degrees:
+            sets.append(cnt[N] % self.sets)
+            cnt[N] += 1
+        return degrees, arrivals, sets
+
+    def outputs(self, degrees, block=None):
+        """(a distinct pinned result buffer per request, the block they lie
+        in): carved from `block` when it is large enough, else from one new
+        allocation, and poisoned so that a response never written cannot
+        pass."""
+        import ncmul
+
+        total = sum(N * self.L for N in degrees)
+        if block is None or block.shape[0] < total:
+            block = ncmul.host_array((total,))
+        block[:total] = 0xFFFFFFFF
+        outs, off = [], 0
+        for N in degrees:
+            outs.append(block[off:off + N * self.L].reshape(N, self.L))
+            off += N * self.L
+        return outs, block
+
+    def replay(self, srv, degrees, arrivals, sets, outs):
+        A = [self.ops[N][0][j] for N, j in zip(degrees, sets)]
+        B = [self.ops[N][1][j] for N, j in zip(degrees, sets)]
+        return srv.replay(degrees, arrivals, A, B, outs)
+
+    def wrong(self, degrees, sets, outs) -> int:
+        """Responses that differ from the plain path's product of their operands."""
+        return sum(not np.array_equal(o, self.want[N][j]) for N, j, o in zip(degrees, sets, outs))
+
+
+def _lat_summary(lat, degrees, arrivals):
+    lat_ms = np.asarray(lat) * 1e3
+    deg = np.asarray(degrees)
+    by_deg = {str(N): {"n": int((deg == N).sum()), "p50_ms": float(np.percentile(lat_ms[deg == N], 50)),
+                       "p99_ms": float(np.percentile(lat_ms[deg == N], 99))} for N in SERVE_DEGREES
+              if (deg == N).any()}
+    # where the slowest 1 % sit in arrival time (tenths of the replay)
+    cut = np.percentile(lat_ms, 99)
+    tenth = np.minimum((np.asarray(arrivals) / arrivals[-1] * 10).astype(int), 9)
+    tail_by_tenth = [int(((lat_ms >= cut) & (tenth == t)).sum()) for t in range(10)]
+    return {"p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(cut), "max_ms": float(lat_ms.max()),
+            "by_degree": by_deg, "slowest_1pct_by_arrival_tenth": tail_by_tenth}
+
+
+def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
+    """C5: n requests, Poisson arrivals at `rate`, replayed open loop into the
+    continuous-batching server (host operands in pinned memory, copies
+    included), every response checked; a rate sweep for the highest offered
+    rate whose p99 stays within p99_bound_ms; a leg whose HBM budget forces
+    admission waits; against the plain form on the same mix (one host-to-host
+    nc_multiply at a time)."""
     import ncmul
 
-    L = q.shape[0]
-    degs = [1 << e for e in range(11, 17)]
-    pin, pout = {}, {}
-    for N in degs:  # request j of a degree uses operands j % 8 and result buffer j % 32 (same operands)
-        pin[N] = []
-        for j in range(8):
-            a, b = ncmul.host_array((N, L)), ncmul.host_array((N, L))
-            a[:] = ncmul.gen_uniform(2 * (20000 + j), N, q)
-            b[:] = ncmul.gen_uniform(2 * (20000 + j) + 1, N, q)
-            pin[N].append((a, b))
-        pout[N] = [ncmul.host_array((N, L)) for _ in range(32)]
-    rng = np.random.default_rng(seed)
-    degrees = [1 << int(x) for x in rng.integers(11, 17, size=n)]
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
-    cnt = {N: 0 for N in degs}
-    A, B, C = [], [], []
-    for N in degrees:
-        j = cnt[N]
-        cnt[N] += 1
-        A.append(pin[N][j % 8][0])
-        B.append(pin[N][j % 8][1])
-        C.append(pout[N][j % 32])
-    srv = ncmul.Server(q, degs, max_batch=32, lanes=2, hbm_budget=16 << 30, autotune=True)
-    # steady state: a short untimed replay of the same mix first (the first
-    # ~100 ms of copies after a quiet spell run slow; see the design notes)
+    mix = ServeMix(q, sets=64)
+    degrees, arrivals, sets = mix.stream(n, rate, seed)
+    outs, block = mix.outputs(degrees)  # later legs carve their buffers from this block
+    srv = ncmul.Server(q, SERVE_DEGREES, max_batch=32, lanes=2, hbm_budget=16 << 30, autotune=True)
+    # steady state: a short untimed replay of another stream of the same mix
+    # first (the first ~100 ms of copies after a quiet spell run slow)
     w = min(n, 1000)
-    srv.replay(degrees[:w], arrivals[:w], A[:w], B[:w], C[:w])
-    lat, makespan = srv.replay(degrees, arrivals, A, B, C)
+    wd, wa, ws = mix.stream(w, rate, seed + 1)
+    mix.replay(srv, wd, wa, ws, mix.outputs(wd, block)[0])
+    outs, _ = mix.outputs(degrees, block)
+    lat, makespan = mix.replay(srv, degrees, arrivals, sets, outs)
     st = srv.stats()
+    wrong = mix.wrong(degrees, sets, outs)
+    # rate sweep on the same server: highest offered rate whose p99 stays
+    # within the bound (each point its own seeded stream, every response checked)
+    sweep, best = [], None
+    sweep_n = min(n, 2000)
+    for r in (5000.0, 6000.0, 7000.0, 8000.0, 9000.0, 10000.0):
+        dg, ar, se = mix.stream(sweep_n, r, seed + int(r))
+        so, _ = mix.outputs(dg, block)
+        lt, mk = mix.replay(srv, dg, ar, se, so)
+        p99 = float(np.percentile(lt, 99) * 1e3)
+        bad = mix.wrong(dg, se, so)
+        sweep.append({"offered_per_s": r, "throughput_per_s": sweep_n / mk,
+                      "p50_ms": float(np.percentile(lt, 50) * 1e3), "p99_ms": p99, "wrong_responses": bad})
+        if p99 <= p99_bound_ms and bad == 0:
+            best = r
     srv.close()
-    plans = {N: ncmul.Plan(N, q) for N in degs}
-    exact = all(np.array_equal(pout[N][j], plans[N].multiply(*pin[N][j % 8])) for N in degs for j in range(32))
+    # admission leg: an HBM budget of two requests at 2^16, far below what
+    # two lanes of 32 per degree would hold, so batches wait for room in the
+    # arena
+    adm_n = min(n, 3000)
+    dg, ar, se = mix.stream(adm_n, 4000.0, seed + 7)
+    with ncmul.Server(q, [65536], max_batch=1, lanes=1, hbm_budget=1 << 40, autotune=False) as probe:
+        f16 = probe.footprint(65536)
+    budget = 2 * f16
+    asrv = ncmul.Server(q, SERVE_DEGREES, max_batch=32, lanes=2, hbm_budget=budget, autotune=True)
+    mix.replay(asrv, dg[:500], ar[:500], se[:500], mix.outputs(dg[:500], block)[0])
+    so, _ = mix.outputs(dg, block)
+    alt, amk = mix.replay(asrv, dg, ar, se, so)
+    ast = asrv.stats()
+    asrv.close()
+    adm = {"hbm_budget": budget, "arena_bytes": ast["device_bytes"], "requests": adm_n, "offered_per_s": 4000.0,
+           "throughput_per_s": adm_n / amk, "p50_ms": float(np.percentile(alt, 50) * 1e3),
+           "p99_ms": float(np.percentile(alt, 99) * 1e3), "admission_waits": ast["admission_waits"],
+           "peak_admitted_bytes": ast["peak_bytes"], "max_batch_seen": ast["max_batch_seen"],
+           "wrong_responses": mix.wrong(dg, se, so)}
+    # plain form on the same mix
+    plans = {N: ncmul.Plan(N, q) for N in SERVE_DEGREES}
     t0, m = time.perf_counter(), 0
     for i in range(min(n, 600)):
         N = degrees[i]
-        plans[N].multiply(*pin[N][0])
+        plans[N].multiply(mix.ops[N][0][sets[i]], mix.ops[N][1][sets[i]])
         m += 1
     plain_rps = m / (time.perf_counter() - t0)
     for p in plans.values():
         p.close()
-    return {
-        "workload": f"{n} requests, N log-uniform in {{2^11..2^16}} (seed {seed}), L=28, Poisson arrivals at "
-                    f"{rate:.0f}/s replayed open loop after an untimed {min(n, 1000)}-request warm-up replay; "
-                    f"host operands (pinned), copies included; continuous "
-                    f"batching, max batch 32, 2 lanes per degree, HBM-footprint admission (16 GiB budget), "
-                    f"per-degree tables resident",
+    out = {
+        "workload": f"{n} requests, N log-uniform in {{2^11..2^16}} (seed {seed}), L=28, 64 operand pairs per degree, "
+                    f"Poisson arrivals at {rate:.0f}/s replayed open loop after an untimed {w}-request warm-up "
+                    f"replay; host operands and a distinct result buffer per request (pinned), copies included; "
+                    f"continuous batching, max batch 32, 2 lanes per degree, HBM-footprint admission (16 GiB "
+                    f"budget), per-degree tables resident",
         "rate_per_s": rate, "throughput_per_s": n / makespan, "makespan_s": makespan,
-        "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "launches": st["launches"], "max_batch_seen": st["max_batch_seen"], "peak_admitted_bytes": st["peak_bytes"],
-        "admission_waits": st["admission_waits"],
+        "arena_bytes": st["device_bytes"], "admission_waits": st["admission_waits"],
+        "responses_checked": n, "wrong_responses": wrong,
         "plain_host_to_host_per_s": plain_rps, "over_plain": (n / makespan) / plain_rps,
-        "bit_exact_vs_plain": bool(exact),
+        "bit_exact_vs_plain": wrong == 0 and all(x["wrong_responses"] == 0 for x in sweep)
+                              and adm["wrong_responses"] == 0,
+        "rate_sweep": {"requests_per_point": sweep_n, "p99_bound_ms": p99_bound_ms, "points": sweep,
+                       "max_rate_within_bound_per_s": best},
+        "admission_leg": adm,
     }
+    out.update(_lat_summary(lat, degrees, arrivals))
+    return out
 
 
 def cpu_baseline(N: int, q: np.ndarray, seconds: float):
@@ -259,6 +391,8 @@ def main() -> int:
     ap.add_argument("--mixed-products", type=int, default=256, help="mixed-degree graph batch size (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=10000, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=6000.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-p99-bound", type=float, default=10.0,
+                    help="p99 bound (ms) of the served rate sweep's capacity figure")
     ap.add_argument("--tuning", default=None,
                     help="force 'width,split' for both plans instead of autotuning (e.g. 30,2)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
@@ -362,12 +496,41 @@ def main() -> int:
     p50 = statistics.median(lat)
     p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
 
-    # bit-exactness of the batched step against the CPU restatement (2 products)
+    # ---- the reference's own 62-bit prime set, same batch, device resident ----
+    p62 = ncmul.Plan(N, q)
+    best62 = None
+    for sp in (2, 3):
+        p62.set_tuning(62, sp)
+        for _ in range(3):
+            p62.multiply_device(d_a, d_b, d_c, B)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            p62.multiply_device(d_a, d_b, d_c, B)
+        torch.cuda.synchronize()
+        ms62 = (time.perf_counter() - t0) / 10 * 1e3
+        if best62 is None or ms62 < best62[1]:
+            best62 = (sp, ms62)
+    p62.set_tuning(62, best62[0])
+    stage62 = p62.profile_stages(d_a, d_b, d_c, B, reps=10, stream=stream)
+    c62 = ncmul.from_device_u32(d_c)
+    p62.close()
+    width62 = {
+        "prime_width": 62, "k": batched.k62, "ntt_split": best62[0], "ms_per_step": best62[1],
+        "products_per_s": B / (best62[1] * 1e-3), "stage_ms": stage62,
+        "ntt_inner_mul32_frac": work_per_stage(N, L, batched.k62, B, best62[0])["ntt_inner"]
+                                / (stage62["ntt_inner"] * 1e-3) / peak_mul,
+        "same_output_as_30_bit": bool(np.array_equal(c62, c_batch)),
+    }
+
+    # bit-exactness of every product of the batched step against the CPU
+    # restatement of the reference (the checker, after the timed region)
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
     oplan = O.Plan(N, q)
-    bit_exact = all(np.array_equal(c_batch[i], oplan.multiply(a[i], b[i])) for i in (0, B - 1))
+    wrong_products = sum(not np.array_equal(c_batch[i], oplan.multiply(a[i], b[i])) for i in range(B))
+    bit_exact = wrong_products == 0
 
     k = batched.k62  # the work convention is fixed on the reference's 62-bit formulation
     work = work_per_stage(N, L, k, B, s1_batch)
@@ -383,6 +546,7 @@ def main() -> int:
     dom = max(stage_ms, key=lambda s: stage_ms[s])
     dom_s = stage_ms[dom] * 1e-3
     traffic, traffic_src = pmc_traffic(dom, N, w_batch, s1_batch, B)
+    t_us, t_src = trace_avg_us(dom, N, w_batch, s1_batch, B)
     roofline = {
         "bound": "valu",
         "kernel": dom,
@@ -396,6 +560,11 @@ def main() -> int:
                            "(W(N) = N [k (6L + 15 log2 N + 29) + 406], k = 29), whatever prime width runs",
         "work_per_launch": work[dom],
         "traffic_source": traffic_src,
+        # the same fraction with the kernel's average duration in the
+        # committed rocprofv3 kernel trace of this configuration
+        "trace_avg_us": t_us,
+        "frac_trace": (work[dom] / (t_us * 1e-6) / peak_mul) if t_us else None,
+        "trace_source": t_src,
     }
     # the same kernel against HBM, in the contract's units: algorithmic
     # (compulsory) bytes per launch / event-timed duration, in GB/s, with the
@@ -405,17 +574,26 @@ def main() -> int:
         "unit": "GB/s", "frac": byts[dom] / dom_s / HBM_PEAK, "traffic": traffic,
         "bytes_per_launch": byts[dom],
     }
-    # every stage against both bounds: compulsory HBM bytes and convention
-    # multiplies per launch over its event-timed duration
+    # every stage against its bounds: compulsory HBM bytes per launch over
+    # its event-timed duration, and the unit its arithmetic runs on -- the
+    # matrix-core stages (decomposition, CRT on the 30-bit set) in issued
+    # int8 operations against the published int8 peak, the VALU stages in
+    # convention multiplies against the measured multiply peak
+    ops8 = int8_ops_per_stage(N, batched.k30, B)
+    mfma_paths = batched.choice(B)[2] if w_batch == 30 else 0
+    on_mfma = {"decompose": bool(mfma_paths & ncmul.MFMA_DECOMPOSE), "crt": bool(mfma_paths & ncmul.MFMA_CRT)}
     stage_roofline = {}
     for st, ms in stage_ms.items():
         if ms <= 0.0:
             continue
         sec = ms * 1e-3
-        stage_roofline[st] = {
-            "ms": ms, "hbm_GBps": byts[st] / sec / 1e9, "hbm_frac": byts[st] / sec / HBM_PEAK,
-            "mul32_frac": work[st] / sec / peak_mul,
-        }
+        e = {"ms": ms, "hbm_GBps": byts[st] / sec / 1e9, "hbm_frac": byts[st] / sec / HBM_PEAK}
+        if on_mfma.get(st):
+            e.update({"unit": "mfma_int8", "int8_ops_per_launch": ops8[st], "int8_TOPS": ops8[st] / sec / 1e12,
+                      "int8_frac": ops8[st] / sec / INT8_PEAK})
+        else:
+            e.update({"unit": "valu_mul32", "mul32_frac": work[st] / sec / peak_mul})
+        stage_roofline[st] = e
     W = N * (k * (6 * L + 15 * (N.bit_length() - 1) + 29) + 406)
     step_total_ms = sum(stage_ms.values())
 
@@ -453,10 +631,12 @@ def main() -> int:
         "plan_create_ms": create_ms,
         "p99_ms": p99,
         "bit_exact": bit_exact,
+        "products_checked": B,
         "stage_ms": stage_ms,
         "roofline": roofline,
         "roofline_hbm": roofline_hbm,
         "stage_roofline": stage_roofline,
+        "width62": width62,
         "w_convention": {
             "W_per_product": W,
             "whole_path_mul32_per_s": batched_pps * W,
@@ -467,11 +647,12 @@ def main() -> int:
     if args.mixed_products:
         out["mixed_batch"] = mixed_batch(q, args.mixed_products, 4, 20)
     if args.serve_requests:
-        out["served"] = served(q, args.serve_requests, args.serve_rate, 12)
+        out["served"] = served(q, args.serve_requests, args.serve_rate, 12, args.serve_p99_bound)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, q, args.cpu_seconds)
     print(json.dumps(out), flush=True)
-    ok = bit_exact and all(out[k]["bit_exact_vs_plain"] for k in ("mixed_batch", "served") if k in out)
+    ok = bit_exact and width62["same_output_as_30_bit"] and all(
+        out[k]["bit_exact_vs_plain"] for k in ("mixed_batch", "served") if k in out)
     return 0 if ok else 3
 
 
