@@ -26,6 +26,7 @@ import os
 import re
 import statistics
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -444,6 +445,7 @@ def main() -> int:
         create_ms[str(n_create)] = min(ts)
     batched = ncmul.Plan(N, q)
     plain = ncmul.Plan(N, q)
+    tuning = None
     if args.tuning:
         w, s1 = (int(x) for x in args.tuning.split(","))
         batched.set_tuning(w, s1)
@@ -451,9 +453,18 @@ def main() -> int:
         w_batch, s1_batch, w_plain, s1_plain = w, s1, w, s1
     else:
         # the batched plan last: its tuning leaves the device busy at the
-        # timed shape right before the warm-up steps
+        # timed shape right before the warm-up steps.  Both tune from scratch
+        # into a fresh cache file; a third plan then takes the batched choice
+        # from that file, as a second process would.
+        cache = Path(tempfile.mkdtemp()) / "tune.txt"
+        ncmul.set_tune_cache(cache)
         w_plain, s1_plain = plain.autotune(1)
         w_batch, s1_batch = batched.autotune(B)
+        with ncmul.Plan(N, q) as again:
+            again.autotune(B)
+            reload = again.tune_info(B)
+        ncmul.set_tune_cache("")  # the other legs tune without it
+        tuning = {"plain": plain.tune_info(1), "batched": batched.tune_info(B), "batched_from_cache": reload}
 
     # ---- batched: the timed steps ----
     for _ in range(args.warmup):
@@ -629,6 +640,7 @@ def main() -> int:
         },
         "batched_over_plain": batched_pps / plain_dev_pps,
         "plan_create_ms": create_ms,
+        "tuning": tuning,
         "p99_ms": p99,
         "bit_exact": bit_exact,
         "products_checked": B,

@@ -2,8 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -86,14 +89,17 @@ Choice heuristic(const nc_plan* p, uint32_t batch, uint32_t width = 30) {
     return c;
 }
 
-Choice choose(const nc_plan* p, uint32_t batch) {
+// unchunked: the configuration for callers that run one pipeline on their
+// own workspace (graph capture, the server), which never chunk.
+Choice choose(const nc_plan* p, uint32_t batch, bool unchunked = false) {
     Choice c;
     const nc_plan::Tuned& t = p->tune[bucket_of(batch)];
     if (p->forced) c = p->forced_choice;
-    else if (t.tuned) c = t.choice;
+    else if (t.tuned) c = unchunked ? t.unchunked : t.choice;
     else c = heuristic(p, batch);
     if (p->mfma_mode >= 0) c.mfma = (uint32_t)p->mfma_mode;
     if (p->chunks_forced) c.chunks = p->chunks_forced;
+    if (unchunked) c.chunks = 1;
     return c;
 }
 
@@ -188,6 +194,92 @@ int ws_release(nc_plan* p, hipStream_t s) {
     return NC_OK;
 }
 
+// ---- persisted autotuner choices ------------------------------------------
+// One text line per tuned (device, shape, batch bucket, pinned settings):
+//   nc-tune-v1 <arch> <cus> <N> <L> <q_bits> <k62> <k30> <bucket> <mfma_mode>
+//   <chunks_forced> : <width> <log_blocks> <mfma> <chunks> <width1>
+//   <log_blocks1> <mfma1> <median_ms>
+// (the second group is the unchunked choice).  The file is appended to; the
+// last line for a key wins.  Path: nc_set_tune_cache, else the environment
+// variable NC_TUNE_CACHE, else no cache.
+std::mutex g_cache_mu;
+std::string g_cache_path;
+bool g_cache_set = false;
+
+std::string cache_path() {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    if (g_cache_set) return g_cache_path;
+    const char* env = std::getenv("NC_TUNE_CACHE");
+    return env ? std::string(env) : std::string();
+}
+
+std::string cache_key(const nc_plan* p, uint32_t batch) {
+    if (cache_path().empty()) return std::string();
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) return std::string();
+    std::string arch(prop.gcnArchName);
+    for (char& ch : arch)
+        if (ch == ' ' || ch == ':') ch = '_';
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "nc-tune-v1 %s %d %u %u %u %u %u %u %d %u", arch.c_str(),
+                  prop.multiProcessorCount, p->host.N, p->host.L, p->host.q_bits, k_of(p, 62), k_of(p, 30),
+                  bucket_of(batch), p->mfma_mode, p->chunks_forced);
+    return buf;
+}
+
+bool valid_choice(const nc_plan* p, const Choice& c) {
+    if (c.width != 30 && c.width != 62) return false;
+    if (c.chunks < 1 || c.chunks > 64 || c.mfma > kMfmaAll) return false;
+    if (p->host.log_n >= 4 &&
+        (c.log_blocks < nc::min_log_blocks(p->host.log_n) || c.log_blocks > nc::max_log_blocks(p->host.log_n)))
+        return false;
+    return true;
+}
+
+std::mutex g_file_mu;  // this process's reads and appends of the cache file
+
+bool cache_lookup(const nc_plan* p, const std::string& key, nc_plan::Tuned* out) {
+    const std::string path = cache_path();
+    std::lock_guard<std::mutex> g(g_file_mu);
+    FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        const char* colon = std::strstr(line, " : ");
+        if (!colon || (size_t)(colon - line) != key.size() || std::strncmp(line, key.c_str(), key.size()) != 0) continue;
+        nc_plan::Tuned t;
+        float med = 0.f;
+        if (std::sscanf(colon + 3, "%u %u %u %u %u %u %u %f", &t.choice.width, &t.choice.log_blocks, &t.choice.mfma,
+                        &t.choice.chunks, &t.unchunked.width, &t.unchunked.log_blocks, &t.unchunked.mfma,
+                        &med) != 8)
+            continue;
+        t.unchunked.chunks = 1;
+        t.median_ms = med;
+        if (!valid_choice(p, t.choice) || !valid_choice(p, t.unchunked)) continue;
+        *out = t;
+        found = true;  // later lines override earlier ones
+    }
+    std::fclose(f);
+    return found;
+}
+
+void cache_store(const std::string& key, const nc_plan::Tuned& t) {
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    std::lock_guard<std::mutex> g(g_file_mu);
+    char buf[512];
+    const int len = std::snprintf(buf, sizeof buf, "%s : %u %u %u %u %u %u %u %.5f\n", key.c_str(), t.choice.width,
+                                  t.choice.log_blocks, t.choice.mfma, t.choice.chunks, t.unchunked.width,
+                                  t.unchunked.log_blocks, t.unchunked.mfma, t.median_ms);
+    // one write of the whole line in append mode: lines from concurrent
+    // processes do not interleave
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;  // an unwritable cache only costs the next process a retune
+    std::fwrite(buf, 1, (size_t)len, f);
+    std::fclose(f);
+}
+
 // Upload one host table vector at `off` of the table allocation.
 template <class V>
 hipError_t upload(char* base, size_t off, const V& v) {
@@ -200,7 +292,7 @@ namespace nc {
 
 int set_error(int status, const std::string& msg) { return fail(status, msg); }
 
-Choice choose_for(const nc_plan* p, uint32_t batch) { return choose(p, batch); }
+Choice choose_for(const nc_plan* p, uint32_t batch) { return choose(p, batch, /*unchunked=*/true); }
 
 size_t workspace_bytes_per_product(const nc_plan* p) { return p->res_bytes; }
 
@@ -598,10 +690,36 @@ int nc_plan_set_mfma(nc_plan* p, int mode) {
     return NC_OK;
 }
 
+int nc_set_tune_cache(const char* path) {
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    g_cache_path = path ? std::string(path) : std::string();
+    g_cache_set = true;
+    return NC_OK;
+}
+
 int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_width, uint32_t* chosen_log_blocks) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
     int rc = check_batch(p, batch);
     if (rc) return rc;
+    const auto wall0 = std::chrono::steady_clock::now();
+    auto wall_ms = [&] {
+        return std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    };
+    nc_plan::Tuned& t = p->tune[bucket_of(batch)];
+    const std::string key = cache_key(p, batch);
+    if (!key.empty()) {
+        nc_plan::Tuned hit;
+        if (cache_lookup(p, key, &hit)) {
+            t = hit;
+            t.tuned = true;
+            t.from_cache = true;
+            t.tune_ms = wall_ms();
+            p->forced = false;
+            if (chosen_width) *chosen_width = t.choice.width;
+            if (chosen_log_blocks) *chosen_log_blocks = t.choice.log_blocks;
+            return NC_OK;
+        }
+    }
     rc = grow_workspace(p, batch);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -618,8 +736,6 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
-    Choice best = heuristic(p, batch);
-    float best_ms = -1.f;
     const uint32_t log_n = p->host.log_n;
     const uint32_t lo = log_n >= 4 ? nc::min_log_blocks(log_n) : 0u;
     const uint32_t hi = log_n >= 4 ? std::max(lo, nc::max_log_blocks(log_n)) : 0u;
@@ -636,22 +752,22 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
         cands.push_back({30u, mfma_default(p)});
     }
     cands.push_back({62u, 0u});
-    // runs of choice c (after one warm-up run): their sum, or their minimum
-    auto time_choice = [&](const Choice& c, int runs, bool min_of_runs = false) {
-        float acc = min_of_runs ? -1.f : 0.f;
+    // one warm-up run of choice c, then `runs` timed runs appended to `out`
+    auto time_runs = [&](const Choice& c, int runs, std::vector<float>& out) {
         for (int rep = 0; rep <= runs && e == hipSuccess; ++rep) {  // rep 0 warms up
             e = hipEventRecord(t0, s);
             if (e == hipSuccess && run_pipeline(p, buf, buf + n, buf + 2 * n, batch, s, c) != NC_OK)
                 e = hipErrorLaunchFailure;
             if (e == hipSuccess) e = hipEventRecord(t1, s);
             if (e == hipSuccess) e = hipEventSynchronize(t1);
-            float t = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
-            if (rep == 0) continue;
-            if (!min_of_runs) acc += t;
-            else if (acc < 0.f || t < acc) acc = t;
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (rep > 0 && e == hipSuccess) out.push_back(ms);
         }
-        return acc;
+    };
+    auto median = [](std::vector<float> v) {
+        std::sort(v.begin(), v.end());
+        return v.empty() ? 0.f : v[v.size() / 2];
     };
     struct Timed {
         float ms;
@@ -664,46 +780,73 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
     if (p->chunks_forced) chunk_opts = {p->chunks_forced};
     for (const Cand& cd : cands) {
         for (uint32_t lb = lo; lb <= hi && e == hipSuccess; ++lb)
-        for (uint32_t ch : chunk_opts) {
-            if (ch > 1 && cd.width != 30 && !p->chunks_forced) continue;
-            Choice c;
-            c.width = cd.width;
-            c.mfma = cd.mfma;
-            c.log_blocks = lb;
-            c.chunks = ch;
-            const float ms = time_choice(c, 2);
-            if (e == hipSuccess) timed.push_back({ms, c});
-        }
+            for (uint32_t ch : chunk_opts) {
+                if (ch > 1 && cd.width != 30 && !p->chunks_forced) continue;
+                Choice c;
+                c.width = cd.width;
+                c.mfma = cd.mfma;
+                c.log_blocks = lb;
+                c.chunks = ch;
+                std::vector<float> runs;
+                time_runs(c, 2, runs);
+                if (e == hipSuccess) timed.push_back({median(runs), c});
+            }
     }
     // The quick pass above times each candidate twice; runs on one device
     // vary by several percent, as much as the leaders differ.  The three
-    // fastest are timed again, in turn, over three rounds, and the lowest
-    // single run decides.
+    // fastest (and the fastest unchunked one, which graphs and the server
+    // use) are timed again, in turn, over three rounds of three runs, and the
+    // lowest median of those nine runs decides: a choice that repeats from
+    // process to process, not one run's luck.
     std::sort(timed.begin(), timed.end(), [](const Timed& x, const Timed& y) { return x.ms < y.ms; });
-    const size_t top = std::min<size_t>(3, timed.size());
-    std::vector<float> best_run(top, -1.f);
+    std::vector<Timed> top(timed.begin(), timed.begin() + std::min<size_t>(3, timed.size()));
+    for (const Timed& x : timed)
+        if (x.c.chunks == 1) {
+            bool in = false;
+            for (const Timed& y : top) in = in || (y.c.width == x.c.width && y.c.log_blocks == x.c.log_blocks &&
+                                                   y.c.mfma == x.c.mfma && y.c.chunks == 1);
+            if (!in) top.push_back(x);
+            break;
+        }
+    std::vector<std::vector<float>> samples(top.size());
     for (int round = 0; round < 3 && e == hipSuccess; ++round)
-        for (size_t i = 0; i < top && e == hipSuccess; ++i) {
-            const float ms = time_choice(timed[i].c, 3, /*min_of_runs=*/true);
-            if (e == hipSuccess && (best_run[i] < 0.f || ms < best_run[i])) best_run[i] = ms;
-        }
-    for (size_t i = 0; i < top; ++i)
-        if (best_run[i] >= 0.f && (best_ms < 0.f || best_run[i] < best_ms)) {
-            best_ms = best_run[i];
-            best = timed[i].c;
-        }
+        for (size_t i = 0; i < top.size() && e == hipSuccess; ++i) time_runs(top[i].c, 3, samples[i]);
+    int best = -1, best1 = -1;
+    std::vector<float> med(top.size());
+    for (size_t i = 0; i < top.size(); ++i) {
+        med[i] = median(samples[i]);
+        if (samples[i].empty()) continue;
+        if (best < 0 || med[i] < med[best]) best = (int)i;
+        if (top[i].c.chunks == 1 && (best1 < 0 || med[i] < med[best1])) best1 = (int)i;
+    }
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (e == hipSuccess) e = hipEventRecord(p->ev_ws, s);
     (void)hipStreamSynchronize(s);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(e, "autotune");
-    nc_plan::Tuned& t = p->tune[bucket_of(batch)];
-    t.choice = best;
+    if (best < 0) return fail(NC_ERR_STATE, "autotune timed no configuration");
+    t.choice = top[best].c;
+    t.unchunked = best1 >= 0 ? top[best1].c : heuristic(p, batch);
+    t.median_ms = med[best];
     t.tuned = true;
+    t.from_cache = false;
+    t.tune_ms = wall_ms();
     p->forced = false;
-    if (chosen_width) *chosen_width = best.width;
-    if (chosen_log_blocks) *chosen_log_blocks = best.log_blocks;
+    if (!key.empty()) cache_store(key, t);
+    if (chosen_width) *chosen_width = t.choice.width;
+    if (chosen_log_blocks) *chosen_log_blocks = t.choice.log_blocks;
+    return NC_OK;
+}
+
+int nc_plan_get_tune_info(const nc_plan* p, uint32_t batch, uint32_t* from_cache, float* tune_ms, float* median_ms) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
+    const nc_plan::Tuned& t = p->tune[bucket_of(batch)];
+    if (!t.tuned) return fail(NC_ERR_STATE, "no tuned configuration for this batch size");
+    if (from_cache) *from_cache = t.from_cache ? 1u : 0u;
+    if (tune_ms) *tune_ms = t.tune_ms;
+    if (median_ms) *median_ms = t.median_ms;
     return NC_OK;
 }
 

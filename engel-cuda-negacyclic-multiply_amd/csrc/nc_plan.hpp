@@ -114,8 +114,12 @@ struct nc_plan {
     // Autotuned choice per batch-size bucket (ceil(log2 batch), 0..31);
     // `forced` pins every bucket (nc_plan_set_tuning).
     struct Tuned {
-        nc::Choice choice;
+        nc::Choice choice;     // nc_multiply_device (may be chunked)
+        nc::Choice unchunked;  // callers that run one pipeline on their own workspace (graphs, server)
         bool tuned = false;
+        bool from_cache = false;
+        float median_ms = 0.f;  // the chosen configuration's median run
+        float tune_ms = 0.f;    // wall time of the nc_plan_autotune call
     };
     Tuned tune[32];
     bool forced = false;

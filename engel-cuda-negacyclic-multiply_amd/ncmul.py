@@ -102,6 +102,9 @@ def lib():
         L.nc_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
         L.nc_plan_autotune.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(ctypes.c_uint32),
                                        ctypes.POINTER(ctypes.c_uint32)]
+        L.nc_set_tune_cache.argtypes = [ctypes.c_char_p]
+        L.nc_plan_get_tune_info.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         L.nc_profile_stages.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
                                         ctypes.POINTER(ctypes.c_float)]
         L.nc_measure_mul32_peak.argtypes = [ctypes.POINTER(ctypes.c_double)]
@@ -168,6 +171,11 @@ def host_twiddles(N: int, q: np.ndarray, width: int, inverse: bool) -> np.ndarra
     out = np.empty(n.value // np.dtype(dt).itemsize, dtype=dt)
     _check(lib().nc_host_twiddles(*args, _vp(out.ctypes.data), ctypes.byref(n)), "nc_host_twiddles")
     return out.reshape(-1, N, 2)
+
+
+def set_tune_cache(path):
+    """Process-wide autotuner cache file (None disables; NC_TUNE_CACHE otherwise)."""
+    _check(lib().nc_set_tune_cache(None if path is None else str(path).encode()), "nc_set_tune_cache")
 
 
 def measure_mul32_peak() -> float:
@@ -265,6 +273,13 @@ class Plan:
                "nc_plan_autotune")
         self.info = self.get_info()
         return int(w.value), int(lb.value)
+
+    def tune_info(self, batch: int) -> dict:
+        """How bucket `batch` was tuned: from_cache, tune_ms, median_ms."""
+        fc, tm, md = ctypes.c_uint32(), ctypes.c_float(), ctypes.c_float()
+        _check(lib().nc_plan_get_tune_info(self._h, batch, ctypes.byref(fc), ctypes.byref(tm), ctypes.byref(md)),
+               "nc_plan_get_tune_info")
+        return {"from_cache": bool(fc.value), "tune_ms": float(tm.value), "median_ms": float(md.value)}
 
     def profile_stages(self, d_a, d_b, d_c, batch: int, reps: int = 5, stream=None):
         """Mean per-launch milliseconds: decompose, ntt_outer_fwd, ntt_inner, ntt_outer_inv, crt."""

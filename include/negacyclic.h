@@ -136,15 +136,33 @@ int nc_plan_set_chunks(nc_plan* plan, uint32_t chunks);
 #define NC_MFMA_CRT 2
 int nc_plan_set_mfma(nc_plan* plan, int mode);
 
-/* Autotuner: times every (prime width, NTT split, matrix-core paths, chunks) for `batch` products on
- * `stream` (zero inputs; the kernels' work does not depend on the data),
- * twice each, then the three fastest again in turn over three rounds, and
- * keeps the one with the lowest single run for batches of that size bucket
- * (ceil(log2 batch)).
- * Clears a setting forced by nc_plan_set_tuning.  Output bits never depend
- * on it.  (The reference has a single fixed schedule, solve.cu:1006-1038.) */
+/* Autotuner: times every (prime width, NTT split, matrix-core paths, chunks)
+ * for `batch` products on `stream` (zero inputs; the kernels' work does not
+ * depend on the data), twice each, then the three fastest and the fastest
+ * unchunked one again in turn over three rounds of three runs, and keeps the
+ * lowest median of those nine runs for batches of that size bucket
+ * (ceil(log2 batch)); the best unchunked configuration is kept as well, for
+ * graphs and the server, which run one pipeline per batch.  With a tuning
+ * cache (nc_set_tune_cache) a choice found there for this device, shape,
+ * bucket and pinned settings is taken without any device work, and a new
+ * choice is appended to it.  Clears a setting forced by nc_plan_set_tuning.
+ * Output bits never depend on it.  (The reference has a single fixed
+ * schedule, solve.cu:1006-1038.) */
 int nc_plan_autotune(nc_plan* plan, uint32_t batch, void* stream, uint32_t* chosen_width,
                      uint32_t* chosen_log_blocks);
+/* Process-wide tuning cache: a text file of one line per tuned (device
+ * architecture and compute-unit count, N, L, bits(q), primes per set, batch
+ * bucket, pinned matrix-core mode and chunks), appended to by every
+ * nc_plan_autotune that times, read by every one that can skip timing.  path
+ * NULL (or "") disables it; without a call the environment variable
+ * NC_TUNE_CACHE names it, if set.  (No counterpart in the reference.) */
+int nc_set_tune_cache(const char* path);
+/* How the tuned configuration of batch-size bucket `batch` came about:
+ * from_cache 1 if nc_plan_autotune took it from the cache, tune_ms the wall
+ * time of that call, median_ms the chosen configuration's median run (any
+ * out pointer may be NULL; NC_ERR_STATE if that bucket is not tuned). */
+int nc_plan_get_tune_info(const nc_plan* plan, uint32_t batch, uint32_t* from_cache, float* tune_ms,
+                          float* median_ms);
 
 /* ---- plain product, host buffers: replaces fherma_run ----------------
  * solve.cu:1489-1528: a, b in host memory (N*L u32 each), c written to host

@@ -453,3 +453,57 @@ def test_errors(gpu, q868):
             plan.set_tuning(62, 9)
         with pytest.raises(ncmul.NcError):
             plan.set_tuning(31)
+
+
+def test_autotune_cache_second_process(gpu, q868, tmp_path):
+    """A choice tuned in one process is appended to the cache file; a second
+    process with the same device and shape takes it without timing anything
+    (from_cache, a tuning call of a few milliseconds), and its products are
+    the oracle's."""
+    import json
+    import os
+    import subprocess
+    import sys
+
+    N, batch = 2048, 8
+    cache = tmp_path / "tune.txt"
+    ncmul.set_tune_cache(cache)
+    try:
+        with ncmul.Plan(N, q868) as plan:
+            plan.autotune(batch)
+            first = plan.tune_info(batch)
+            choice = plan.choice(batch)
+    finally:
+        ncmul.set_tune_cache(None)
+    say("first tuning", first, choice, cache.read_text())
+    assert not first["from_cache"] and cache.read_text().count("\n") == 1
+    child = r'''
+import json, sys
+import numpy as np
+sys.path[:0] = sys.argv[1:3]
+import torch
+torch.cuda.init()  # torch's device runtime first, as in every other entry point
+import ncmul, oracle as O
+q = np.frombuffer(bytes.fromhex(sys.argv[3]), dtype="<u4").astype(np.uint32)
+N, batch = 2048, 8
+with ncmul.Plan(N, q) as plan:
+    plan.autotune(batch)
+    info, choice = plan.tune_info(batch), plan.choice(batch)
+    A = np.stack([ncmul.gen_uniform(900 + 2 * i, N, q) for i in range(batch)])
+    B = np.stack([ncmul.gen_uniform(901 + 2 * i, N, q) for i in range(batch)])
+    d_c = ncmul.empty_device_u32((batch, N, 28))
+    plan.multiply_device(ncmul.to_device_u32(A), ncmul.to_device_u32(B), d_c, batch)
+    got = ncmul.from_device_u32(d_c)
+    op = O.Plan(N, q)
+    exact = all(np.array_equal(got[i], op.multiply(A[i], B[i])) for i in range(batch))
+print(json.dumps({"info": info, "choice": list(choice), "exact": exact}))
+'''
+    from conftest import PKG, ROOT
+    env = dict(os.environ, NC_TUNE_CACHE=str(cache))
+    r = subprocess.run([sys.executable, "-c", child, str(PKG), str(ROOT / "oracle"), q868.astype("<u4").tobytes().hex()],
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    say("second process", got)
+    assert got["info"]["from_cache"] and got["info"]["tune_ms"] < 50.0
+    assert tuple(got["choice"]) == choice and got["exact"]
