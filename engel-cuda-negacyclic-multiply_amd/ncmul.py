@@ -9,12 +9,15 @@ back to anything else.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
-LIB_PATH = PKG_DIR / "lib" / "libnegacyclic.so"
+# NC_LIB_PATH: another build of the same library (the sanitizer build,
+# lib/libnegacyclic_asan.so, that tests/test_sanitize.py loads)
+LIB_PATH = Path(os.environ["NC_LIB_PATH"]) if os.environ.get("NC_LIB_PATH") else PKG_DIR / "lib" / "libnegacyclic.so"
 HEADER = PKG_DIR.parent / "include" / "negacyclic.h"
 
 NC_OK = 0

@@ -21,7 +21,9 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "libncoracle.so"
+# NC_ORACLE_LIB: another build of the oracle (the sanitizer build,
+# libncoracle_asan.so, that tests/test_sanitize.py loads)
+LIB_PATH = Path(os.environ["NC_ORACLE_LIB"]) if os.environ.get("NC_ORACLE_LIB") else HERE / "libncoracle.so"
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
