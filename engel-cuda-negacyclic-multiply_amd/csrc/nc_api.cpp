@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -316,6 +317,30 @@ int validate_batch(const nc_plan* p, uint32_t batch) { return check_batch(p, bat
 
 }  // namespace nc
 
+// One table set for every degree of one modulus (negacyclic_batch.h): the
+// master tables of its largest degree, resident on one device, shared by the
+// plans created from it (each holds a reference, as does the creator).
+struct nc_tableset {
+    nc::HostTables host;
+    nc::DevPlan dev{};
+    int device = 0;
+    void* d_tables = nullptr;
+    size_t table_bytes = 0;
+    float create_ms = 0.f;
+    std::atomic<int> refs{1};
+};
+
+namespace {
+
+void tableset_release(nc_tableset* ts) {
+    if (ts && ts->refs.fetch_sub(1) == 1) {
+        if (ts->d_tables) (void)hipFree(ts->d_tables);
+        delete ts;
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* nc_status_string(int status) {
@@ -357,6 +382,7 @@ void nc_plan_destroy(nc_plan* p) {
         (void)hipEventSynchronize(p->ev_ws);
         (void)hipEventDestroy(p->ev_ws);
     }
+    tableset_release(p->shared);
     delete p;
 }
 
@@ -364,8 +390,9 @@ namespace {
 
 // Sampled entries of the device-built twiddle tables (both directions,
 // both sets: first, middle and last prime; x = 1, 2, N / 2 + 1, N - 1 and
-// pseudo-random x) against host arithmetic.
-int check_device_twiddles(const nc::HostTables& t, const char* f62, const char* i62, const char* f30,
+// pseudo-random x < N) against host arithmetic; rows are 2^tw_log entries
+// apart (a shared set's rows are longer than N: degree N reads their prefix).
+int check_device_twiddles(const nc::HostTables& t, uint32_t tw_log, const char* f62, const char* i62, const char* f30,
                           const char* i30) {
     const uint32_t k62 = (uint32_t)t.s62.primes.size(), k30 = (uint32_t)t.s30.primes.size(), N = t.N;
     uint64_t lcg = 0x9e3779b97f4a7c15ull;
@@ -379,7 +406,7 @@ int check_device_twiddles(const nc::HostTables& t, const char* f62, const char* 
             }
             for (uint32_t x : xs) {
                 for (int inv = 0; inv < 2; ++inv) {
-                    const size_t at = (size_t)j * N + x;
+                    const size_t at = ((size_t)j << tw_log) + x;
                     bool same;
                     hipError_t e;
                     if (set == 0) {
@@ -403,62 +430,16 @@ int check_device_twiddles(const nc::HostTables& t, const char* f62, const char* 
     return NC_OK;
 }
 
-}  // namespace
-
-int nc_check_tables(uint32_t N, uint32_t L, const uint32_t* q) {
-    if (!q) return fail(NC_ERR_INVALID, "q is null");
-    try {
-        nc::self_test(nc::build_tables(N, L, q, /*host_twiddles=*/false));
-    } catch (const std::invalid_argument& e) {
-        return fail(NC_ERR_INVALID, e.what());
-    } catch (const std::runtime_error& e) {
-        const std::string m = e.what();
-        return fail(m.rfind("self test", 0) == 0 ? NC_ERR_SELFTEST : NC_ERR_PRIMES, m);
-    } catch (const std::bad_alloc&) {
-        return fail(NC_ERR_NOMEM, "host tables");
-    }
-    return NC_OK;
-}
-
-int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
-    if (!out) return fail(NC_ERR_INVALID, "out is null");
-    *out = nullptr;
-    if (!q) return fail(NC_ERR_INVALID, "q is null");
-    nc_plan* p = new (std::nothrow) nc_plan();
-    if (!p) return fail(NC_ERR_NOMEM, "plan allocation");
-    try {
-        p->host = nc::build_tables(N, L, q, /*host_twiddles=*/false);
-        nc::self_test(p->host);
-    } catch (const std::invalid_argument& e) {
-        delete p;
-        return fail(NC_ERR_INVALID, e.what());
-    } catch (const std::runtime_error& e) {
-        delete p;
-        const std::string m = e.what();
-        return fail(m.rfind("self test", 0) == 0 ? NC_ERR_SELFTEST : NC_ERR_PRIMES, m);
-    } catch (const std::bad_alloc&) {
-        delete p;
-        return fail(NC_ERR_NOMEM, "host tables");
-    }
-    const nc::HostTables& t = p->host;
-    auto cleanup = [&](int rc) {
-        nc_plan_destroy(p);
-        return rc;
-    };
-    hipError_t e = hipGetDevice(&p->device);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
-    // the allocation-free entry point's workspace: this device's scratch
-    // array, looked up once per device
-    e = nc::scratch_address(&p->scratch);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "scratch address"));
-
-    // one resident allocation for both prime sets' tables
+// Host tables -> one resident device allocation: every table of both prime
+// sets, the [k][N] twiddle tables built in place on the device and checked
+// against host arithmetic; fills the device view d (strides tw_log = log N).
+int upload_tables(const nc::HostTables& t, int device, void** d_tables, size_t* bytes, nc::DevPlan& d) {
     const nc::HostSet62& a = t.s62;
     const nc::HostSet30& b = t.s30;
     size_t off = 0;
-    auto slot = [&](size_t bytes) {
+    auto slot = [&](size_t n) {
         const size_t at = off;
-        off = align_up(off + bytes, 256);
+        off = align_up(off + n, 256);
         return at;
     };
     const size_t k62 = a.primes.size(), k30 = b.primes.size();
@@ -475,10 +456,9 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     const size_t o30_mt = slot(b.mfma_tab.size()), o30_mg = slot(b.mfma_bias.size() * sizeof(int32_t));
     const size_t o30_ct = slot(b.crt_tab.size()), o30_cc2 = slot(b.crt_c.size() * sizeof(int64_t));
     const size_t o30_cm = slot(sizeof(nc::CrtMfmaConst));
-    p->table_bytes = off;
-    e = hipMalloc(&p->d_tables, off);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "table allocation"));
-    char* base = static_cast<char*>(p->d_tables);
+    *bytes = off;
+    NC_HIP(hipMalloc(d_tables, off), "table allocation");
+    char* base = static_cast<char*>(*d_tables);
     const hipError_t ups[] = {
         upload(base, o62_pow, a.tw_pow), upload(base, o62_pc, a.pc),
         upload(base, o62_pl, a.place),   upload(base, o62_cc, a.cc),      upload(base, o62_mq, a.miq),
@@ -491,25 +471,27 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         hipMemcpy(base + o30_cm, &b.crt_mc, sizeof(nc::CrtMfmaConst), hipMemcpyHostToDevice),
     };
     for (hipError_t u : ups)
-        if (u != hipSuccess) return cleanup(hip_fail(u, "table upload"));
+        if (u != hipSuccess) return hip_fail(u, "table upload");
     // the [k][N] twiddle tables, built in place on the device
-    e = nc::launch_twiddles62(reinterpret_cast<nc::Tw*>(base + o62_twf), reinterpret_cast<nc::Tw*>(base + o62_twi),
-                              reinterpret_cast<const nc::Tw*>(base + o62_pow),
-                              reinterpret_cast<const nc::PrimeConst*>(base + o62_pc), t.log_n, (uint32_t)k62, nullptr);
+    hipError_t e = nc::launch_twiddles62(reinterpret_cast<nc::Tw*>(base + o62_twf),
+                                         reinterpret_cast<nc::Tw*>(base + o62_twi),
+                                         reinterpret_cast<const nc::Tw*>(base + o62_pow),
+                                         reinterpret_cast<const nc::PrimeConst*>(base + o62_pc), t.log_n,
+                                         (uint32_t)k62, nullptr);
     if (e == hipSuccess)
         e = nc::launch_twiddles30(
             reinterpret_cast<nc::Tw32*>(base + o30_twf), reinterpret_cast<nc::Tw32*>(base + o30_twi),
             reinterpret_cast<const nc::Tw32*>(base + o30_pow), reinterpret_cast<const nc::PrimeConst32*>(base + o30_pc),
             t.log_n, (uint32_t)k30, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle tables"));
-    if (int rc = check_device_twiddles(t, base + o62_twf, base + o62_twi, base + o30_twf, base + o30_twi))
-        return cleanup(rc);
-    nc::DevPlan& d = p->dev;
+    if (e != hipSuccess) return hip_fail(e, "twiddle tables");
+    if (int rc = check_device_twiddles(t, t.log_n, base + o62_twf, base + o62_twi, base + o30_twf, base + o30_twi))
+        return rc;
     d.N = t.N;
     d.L = t.L;
     d.log_n = t.log_n;
     d.w62.k = (uint32_t)a.primes.size();
+    d.w62.tw_log = d.w30.tw_log = t.log_n;  // these [k][N] tables
     d.w62.tw_fwd = reinterpret_cast<const nc::Tw*>(base + o62_twf);
     d.w62.tw_inv = reinterpret_cast<const nc::Tw*>(base + o62_twi);
     d.w62.pc = reinterpret_cast<const nc::PrimeConst*>(base + o62_pc);
@@ -538,13 +520,21 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
         d.w30.crt_kc = b.crt_kc;
     }
     int cus = 0;
-    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "device attribute"));
+    NC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     d.cus = cus > 0 ? (uint32_t)cus : 1u;
-    p->res_bytes = std::max((size_t)d.w62.k * 8, (size_t)d.w30.k * 4) * t.N;
+    return NC_OK;
+}
 
+// The rest of a plan once its tables are resident: the scratch address of
+// the allocation-free entry point, streams and events, host-path staging and
+// a one-product workspace.
+int finish_plan(nc_plan* p) {
+    const nc::HostTables& t = p->host;
+    hipError_t e = nc::scratch_address(&p->scratch);  // this device's scratch array, looked up once per device
+    if (e != hipSuccess) return hip_fail(e, "scratch address");
+    p->res_bytes = std::max((size_t)p->dev.w62.k * 8, (size_t)p->dev.w30.k * 4) * t.N;
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "stream"));
+    if (e != hipSuccess) return hip_fail(e, "stream");
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
         e = hipStreamCreateWithFlags(&p->sub[k], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join[k], hipEventDisableTiming);
@@ -554,14 +544,145 @@ int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_ws, hipEventDisableTiming);
     for (int i = 0; i < nc_plan::kHostPieces && e == hipSuccess; ++i)
         e = hipEventCreateWithFlags(&p->ev_piece[i], hipEventDisableTiming);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "chunk streams"));
+    if (e != hipSuccess) return hip_fail(e, "chunk streams");
     const size_t io = (size_t)t.N * t.L * sizeof(uint32_t);
-    e = hipMalloc(&p->d_io, 3 * io);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "staging"));
-    e = hipHostMalloc(reinterpret_cast<void**>(&p->h_pin), 3 * io, hipHostMallocDefault);
-    if (e != hipSuccess) return cleanup(hip_fail(e, "pinned staging"));
-    int rc = grow_workspace(p, 1);
-    if (rc) return cleanup(rc);
+    NC_HIP(hipMalloc(&p->d_io, 3 * io), "staging");
+    NC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_pin), 3 * io, hipHostMallocDefault), "pinned staging");
+    return grow_workspace(p, 1);
+}
+
+// Runs a host-table step, mapping its exceptions to status codes.
+extern "C++" template <class F>
+int guarded_tables(F&& f) {
+    try {
+        f();
+    } catch (const std::invalid_argument& e) {
+        return fail(NC_ERR_INVALID, e.what());
+    } catch (const std::runtime_error& e) {
+        const std::string m = e.what();
+        return fail(m.rfind("self test", 0) == 0 ? NC_ERR_SELFTEST : NC_ERR_PRIMES, m);
+    } catch (const std::bad_alloc&) {
+        return fail(NC_ERR_NOMEM, "host tables");
+    }
+    return NC_OK;
+}
+
+}  // namespace
+
+int nc_check_tables(uint32_t N, uint32_t L, const uint32_t* q) {
+    if (!q) return fail(NC_ERR_INVALID, "q is null");
+    return guarded_tables([&] { nc::self_test(nc::build_tables(N, L, q, /*host_twiddles=*/false)); });
+}
+
+int nc_plan_create(uint32_t N, uint32_t L, const uint32_t* q, nc_plan** out) {
+    if (!out) return fail(NC_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (!q) return fail(NC_ERR_INVALID, "q is null");
+    nc_plan* p = new (std::nothrow) nc_plan();
+    if (!p) return fail(NC_ERR_NOMEM, "plan allocation");
+    int rc = guarded_tables([&] {
+        p->host = nc::build_tables(N, L, q, /*host_twiddles=*/false);
+        nc::self_test(p->host);
+    });
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    const hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
+    if (rc == NC_OK) rc = upload_tables(p->host, p->device, &p->d_tables, &p->table_bytes, p->dev);
+    if (rc == NC_OK) rc = finish_plan(p);
+    if (rc) {
+        nc_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return NC_OK;
+}
+
+int nc_tableset_create(uint32_t L, const uint32_t* q, nc_tableset** out) {
+    if (!out) return fail(NC_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (!q) return fail(NC_ERR_INVALID, "q is null");
+    const auto t0 = std::chrono::steady_clock::now();
+    nc_tableset* ts = new (std::nothrow) nc_tableset();
+    if (!ts) return fail(NC_ERR_NOMEM, "table set allocation");
+    int rc = guarded_tables([&] {
+        ts->host = nc::build_tables(1u << nc::kMaxLogN, L, q, /*host_twiddles=*/false);
+        nc::self_test(ts->host);
+    });
+    hipError_t e = hipSuccess;
+    if (rc == NC_OK) e = hipGetDevice(&ts->device);
+    if (rc == NC_OK && e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
+    if (rc == NC_OK) rc = upload_tables(ts->host, ts->device, &ts->d_tables, &ts->table_bytes, ts->dev);
+    if (rc) {
+        tableset_release(ts);
+        return rc;
+    }
+    ts->create_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = ts;
+    return NC_OK;
+}
+
+void nc_tableset_destroy(nc_tableset* ts) { tableset_release(ts); }
+
+int nc_tableset_get_info(const nc_tableset* ts, uint32_t* k62, uint32_t* k30, uint64_t* table_bytes,
+                         float* create_ms) {
+    if (!ts) return fail(NC_ERR_INVALID, "null table set");
+    if (k62) *k62 = ts->dev.w62.k;
+    if (k30) *k30 = ts->dev.w30.k;
+    if (table_bytes) *table_bytes = ts->table_bytes;
+    if (create_ms) *create_ms = ts->create_ms;
+    return NC_OK;
+}
+
+int nc_plan_create_shared(nc_tableset* ts, uint32_t N, nc_plan** out) {
+    if (!out) return fail(NC_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (!ts) return fail(NC_ERR_INVALID, "null table set");
+    int dev = 0;
+    NC_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != ts->device) return fail(NC_ERR_INVALID, "the table set lives on another device");
+    nc_plan* p = new (std::nothrow) nc_plan();
+    if (!p) return fail(NC_ERR_NOMEM, "plan allocation");
+    int rc = guarded_tables([&] {
+        p->host = nc::derive_tables(ts->host, N);
+        nc::self_test(p->host);
+    });
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    ts->refs.fetch_add(1);
+    p->shared = ts;
+    p->device = dev;
+    // the plan's own device tables: the N-dependent per-prime constants of both sets
+    const nc::HostTables& t = p->host;
+    const size_t b62 = align_up(t.s62.pc.size() * sizeof(nc::PrimeConst), 256);
+    const size_t b30 = t.s30.pc.size() * sizeof(nc::PrimeConst32);
+    p->table_bytes = b62 + b30;
+    hipError_t e = hipMalloc(&p->d_tables, p->table_bytes);
+    char* base = static_cast<char*>(p->d_tables);
+    if (e == hipSuccess) e = upload(base, 0, t.s62.pc);
+    if (e == hipSuccess) e = upload(base, b62, t.s30.pc);
+    if (e != hipSuccess) rc = hip_fail(e, "plan constants");
+    if (rc == NC_OK) {
+        p->dev = ts->dev;  // the set's tables, rows 2^16 entries apart
+        p->dev.N = t.N;
+        p->dev.log_n = t.log_n;
+        p->dev.w62.pc = reinterpret_cast<const nc::PrimeConst*>(base);
+        p->dev.w30.pc = reinterpret_cast<const nc::PrimeConst32*>(base + b62);
+        // degree N reads the prefix of every row: sampled entries against host arithmetic at N
+        rc = check_device_twiddles(t, ts->dev.w30.tw_log, reinterpret_cast<const char*>(ts->dev.w62.tw_fwd),
+                                   reinterpret_cast<const char*>(ts->dev.w62.tw_inv),
+                                   reinterpret_cast<const char*>(ts->dev.w30.tw_fwd),
+                                   reinterpret_cast<const char*>(ts->dev.w30.tw_inv));
+    }
+    if (rc == NC_OK) rc = finish_plan(p);
+    if (rc) {
+        nc_plan_destroy(p);
+        return rc;
+    }
     *out = p;
     return NC_OK;
 }
@@ -604,7 +725,10 @@ int nc_plan_copy_twiddles(const nc_plan* p, uint32_t width, int inverse, void* o
         need = n * sizeof(nc::Tw32);
     }
     if (bytes != need) return fail(NC_ERR_SIZE, "bytes must equal the twiddle table size");
-    NC_HIP(hipMemcpy(out, src, need, hipMemcpyDeviceToHost), "twiddle copy");
+    // rows of N entries, 2^tw_log entries apart (a shared set's rows are longer)
+    const size_t entry = need / n, row = entry * p->host.N;
+    const size_t pitch = entry << (width == 62 ? p->dev.w62.tw_log : p->dev.w30.tw_log);
+    NC_HIP(hipMemcpy2D(out, row, src, pitch, row, k_of(p, width), hipMemcpyDeviceToHost), "twiddle copy");
     return NC_OK;
 }
 

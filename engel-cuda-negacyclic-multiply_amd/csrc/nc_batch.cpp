@@ -1,6 +1,7 @@
 // Batches of mixed ring degree as one HIP graph (include/negacyclic_batch.h).
 //
-// Each degree keeps one resident plan (tables for that N).  A capture forks
+// Each degree keeps one resident plan on the batch's one table set (the
+// tables of every degree; each plan owns only its N-dependent constants).  A capture forks
 // one stream per distinct degree from the capture stream, enqueues every
 // group of that degree on it with the configuration its plan holds for the
 // group's count, and joins them back: the graph has one branch per degree,
@@ -21,6 +22,7 @@
 struct nc_batch {
     uint32_t L = 0;
     std::vector<uint32_t> q;
+    nc_tableset* tables = nullptr;  // one table set for every degree, made with the first plan
     std::map<uint32_t, nc_plan*> plans;
     std::map<uint32_t, hipStream_t> branch;  // capture branch per degree
     hipStream_t capture = nullptr;
@@ -83,6 +85,7 @@ void nc_batch_destroy(nc_batch* b) {
     for (auto& kv : b->branch) (void)hipStreamDestroy(kv.second);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     for (auto& kv : b->plans) nc_plan_destroy(kv.second);
+    nc_tableset_destroy(b->tables);
     delete b;
 }
 
@@ -90,8 +93,10 @@ int nc_batch_plan(nc_batch* b, uint32_t N, nc_plan** out) {
     if (!b || !out) return set_error(NC_ERR_INVALID, "null argument");
     auto it = b->plans.find(N);
     if (it == b->plans.end()) {
+        if (!b->tables)
+            if (int rc = nc_tableset_create(b->L, b->q.data(), &b->tables)) return rc;
         nc_plan* p = nullptr;
-        if (int rc = nc_plan_create(N, b->L, b->q.data(), &p)) return rc;
+        if (int rc = nc_plan_create_shared(b->tables, N, &p)) return rc;
         it = b->plans.emplace(N, p).first;
     }
     *out = it->second;

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
     const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
     uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int32_t* __restrict__ bias,
     const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
-    uint32_t tiles, uint32_t nb) {
+    uint32_t tiles, uint32_t nb, uint32_t tw_log) {
     constexpr int G = 1 << S1;                 // spread tiles per column group
     constexpr int NG = kMfmaCoeffTiles / G;    // column groups per wave
     static_assert(NG >= 1, "at most 4 spread tiles");
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
         lds_prime[j] = j < k ? PrimeLds{pc[j].p, pc[j].mont_neg} : PrimeLds{1u, 0u};
         if constexpr (S1 > 0) {
 #pragma unroll
-            for (int x = 1; x < G; ++x) lds_tw[j][x] = j < k ? twf[(size_t)j * N + x] : Tw32{0u, 0u};
+            for (int x = 1; x < G; ++x) lds_tw[j][x] = j < k ? twf[((size_t)j << tw_log) + x] : Tw32{0u, 0u};
         }
     }
     __syncthreads();
@@ -515,7 +515,7 @@ template <class A, int LOGB>
 __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
     typename A::T* __restrict__ ra, const typename A::T* __restrict__ rb, const typename A::Twid* __restrict__ twf,
     const typename A::Twid* __restrict__ twi, const typename A::Prime* __restrict__ pcs, uint32_t log_n, uint32_t k,
-    uint32_t nprod) {
+    uint32_t nprod, uint32_t tw_log) {
     using S = InnerShape<LOGB>;
     using T = typename A::T;
     constexpr int LAST = S::NP - 1;
@@ -539,8 +539,8 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
     const size_t off = ((size_t)prod * k + j) * N + ((size_t)blk << LOGB);
     const uint32_t gblk = blk << LOGB;
     const typename A::Prime c = pcs[j];
-    const typename A::Twid* tf = twf + (size_t)j * N;
-    const typename A::Twid* ti = twi + (size_t)j * N;
+    const typename A::Twid* tf = twf + ((size_t)j << tw_log);
+    const typename A::Twid* ti = twi + ((size_t)j << tw_log);
     const bool final_out = (nblk_log == 0);
 
     using P0 = PassMap<LOGB, 0>;
@@ -695,7 +695,8 @@ __device__ __forceinline__ void inverse30_from(uint32_t (&v)[16], uint32_t* lds,
 template <int LOGB>
 __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     uint32_t* __restrict__ ra, const uint32_t* __restrict__ rb, const Tw32* __restrict__ twf,
-    const Tw32* __restrict__ twi, const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t k, uint32_t nprod) {
+    const Tw32* __restrict__ twi, const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t k, uint32_t nprod,
+    uint32_t tw_log) {
     using S = InnerShape<LOGB>;
     using T = uint32_t;
     constexpr int LAST = S::NP - 1;
@@ -720,8 +721,8 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     const size_t off = ((size_t)prod * k + j) * N + ((size_t)blk << LOGB);
     const uint32_t gblk = blk << LOGB;
     const PrimeConst32 c = pcs[j];
-    const Tw32* tf = twf + (size_t)j * N;
-    const Tw32* ti = twi + (size_t)j * N;
+    const Tw32* tf = twf + ((size_t)j << tw_log);
+    const Tw32* ti = twi + ((size_t)j << tw_log);
     const bool final_out = (nblk_log == 0);
     using P0 = PassMap<LOGB, 0>;  // lane-contiguous: x = tid + e TPB
     T* ga0 = ra + off + P0::base(tid);
@@ -760,7 +761,8 @@ __global__ __launch_bounds__(256) void ntt_outer_fwd_kernel(typename A::T* __res
                                                             typename A::T* __restrict__ rb,
                                                             const typename A::Twid* __restrict__ twf,
                                                             const typename A::Prime* __restrict__ pcs,
-                                                            uint32_t log_n, uint32_t k, uint32_t nprod) {
+                                                            uint32_t log_n, uint32_t k, uint32_t nprod,
+                                                            uint32_t tw_log) {
     using T = typename A::T;
     constexpr int NE = 1 << S1;
     const uint32_t b = log_n - S1;
@@ -778,7 +780,7 @@ __global__ __launch_bounds__(256) void ntt_outer_fwd_kernel(typename A::T* __res
     T v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) v[e] = base[(size_t)e << b];
-    ct_pass<A, S1, NE>(v, col, (int)b, (int)log_n, twf + ((size_t)j << log_n), c.p, c.p2);
+    ct_pass<A, S1, NE>(v, col, (int)b, (int)log_n, twf + ((size_t)j << tw_log), c.p, c.p2);
 #pragma unroll
     for (int e = 0; e < NE; ++e) base[(size_t)e << b] = v[e];
 }
@@ -787,7 +789,8 @@ template <class A, int S1>
 __global__ __launch_bounds__(256) void ntt_outer_inv_kernel(typename A::T* __restrict__ ra,
                                                             const typename A::Twid* __restrict__ twi,
                                                             const typename A::Prime* __restrict__ pcs,
-                                                            uint32_t log_n, uint32_t k, uint32_t nprod) {
+                                                            uint32_t log_n, uint32_t k, uint32_t nprod,
+                                                            uint32_t tw_log) {
     using T = typename A::T;
     constexpr int NE = 1 << S1;
     const uint32_t b = log_n - S1;
@@ -803,7 +806,7 @@ __global__ __launch_bounds__(256) void ntt_outer_inv_kernel(typename A::T* __res
     T v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) v[e] = base[(size_t)e << b];
-    gs_pass<A, S1, NE>(v, col, (int)b, (int)log_n, twi + ((size_t)j << log_n), c);
+    gs_pass<A, S1, NE>(v, col, (int)b, (int)log_n, twi + ((size_t)j << tw_log), c);
 #pragma unroll
     for (int e = 0; e < NE; ++e) base[(size_t)e << b] = A::sub_if_ge(v[e], c.p);
 }
@@ -817,7 +820,7 @@ __global__ __launch_bounds__(64) void ntt_tiny_kernel(typename A::T* __restrict_
                                                       const typename A::Twid* __restrict__ twf,
                                                       const typename A::Twid* __restrict__ twi,
                                                       const typename A::Prime* __restrict__ pcs, uint32_t log_n,
-                                                      uint32_t k, uint32_t nprod) {
+                                                      uint32_t k, uint32_t nprod, uint32_t tw_log) {
     using T = typename A::T;
     const uint32_t t = blockIdx.x * 64 + threadIdx.x;
     if (t >= k * nprod) return;
@@ -825,8 +828,8 @@ __global__ __launch_bounds__(64) void ntt_tiny_kernel(typename A::T* __restrict_
     const uint32_t N = 1u << log_n;
     const size_t off = ((size_t)prod * k + j) * N;
     const typename A::Prime c = pcs[j];
-    const typename A::Twid* tf = twf + (size_t)j * N;
-    const typename A::Twid* ti = twi + (size_t)j * N;
+    const typename A::Twid* tf = twf + ((size_t)j << tw_log);
+    const typename A::Twid* ti = twi + ((size_t)j << tw_log);
     T x[8], y[8];
     for (uint32_t i = 0; i < N; ++i) {
         x[i] = ra[off + i];
@@ -1165,7 +1168,7 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
                                                 const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
                                                 uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n,
                                                 uint32_t nprod, const PrimeConst32* __restrict__ pcs = nullptr,
-                                                const Tw32* __restrict__ twi = nullptr) {
+                                                const Tw32* __restrict__ twi = nullptr, uint32_t tw_log = 0) {
     static_assert(INV == 0 || INV == 2, "fused inverse stages: 0 or 2");
     constexpr int LF = 8 * RT - 4;                         // fraction pseudo-limbs LF, LF + 1
     constexpr int LQ = LF < kMaxLimbs ? LF : kMaxLimbs;  // q limbs (zero beyond L)
@@ -1188,7 +1191,7 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             const uint32_t r = r2 >> 1, u = r2 & 1u;
             const uint32_t j = r < k ? r : k - 1;
             const PrimeConst32 c = pcs[j];
-            const Tw32 t = twi[((size_t)j << log_n) + 2 + u];
+            const Tw32 t = twi[((size_t)j << tw_log) + 2 + u];
             lds_inv[r2] = u ? CrtInvLds{c.p, t.w, t.ws, c.wr, c.wr_s, 2 * c.p + 1, 0u, 0u}
                             : CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, 0u, 0u, 0u};
         }
@@ -1399,8 +1402,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     const uint32_t* __restrict__ res, uint32_t* __restrict__ out, const int8_t* __restrict__ tab,
     const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
     uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod, const PrimeConst32* __restrict__ pcs,
-    const Tw32* __restrict__ twi) {
-    crt30_mfma_body<RT, KCT, LT, INV>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod, pcs, twi);
+    const Tw32* __restrict__ twi, uint32_t tw_log) {
+    crt30_mfma_body<RT, KCT, LT, INV>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod, pcs, twi, tw_log);
 }
 
 template <int RT>
@@ -1546,10 +1549,10 @@ static void launch_inner(const DevPlan& d, void* ra, const void* rb, uint32_t np
     const uint32_t grid = (items + S::BPW - 1) / S::BPW;
     if constexpr (std::is_same<A, W30>::value && LOGB >= 5)
         ntt_inner30_kernel<LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                         set.tw_inv, set.pc, d.log_n, set.k, nprod);
+                                                         set.tw_inv, set.pc, d.log_n, set.k, nprod, set.tw_log);
     else
         ntt_inner_kernel<A, LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                          set.tw_inv, set.pc, d.log_n, set.k, nprod);
+                                                          set.tw_inv, set.pc, d.log_n, set.k, nprod, set.tw_log);
 }
 
 template <class A>
@@ -1560,7 +1563,7 @@ static hipError_t inner_dispatch(const DevPlan& d, void* ra, const void* rb, uin
     if (d.log_n < 4) {
         const uint32_t t = set.k * nprod;
         ntt_tiny_kernel<A><<<(t + 63) / 64, 64, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                        set.tw_inv, set.pc, d.log_n, set.k, nprod);
+                                                        set.tw_inv, set.pc, d.log_n, set.k, nprod, set.tw_log);
         return hipGetLastError();
     }
     switch (d.log_n - log_blocks) {
@@ -1587,7 +1590,7 @@ static void launch_outer_fwd(const DevPlan& d, void* ra, void* rb, uint32_t npro
     const uint32_t wgs_per_row = ((d.N >> S1) + 255) / 256;
     const uint32_t grid = wgs_per_row * 2 * set.k * nprod;
     ntt_outer_fwd_kernel<A, S1><<<grid, 256, 0, s>>>(static_cast<T*>(ra), static_cast<T*>(rb), set.tw_fwd, set.pc,
-                                                      d.log_n, set.k, nprod);
+                                                      d.log_n, set.k, nprod, set.tw_log);
 }
 
 template <class A, int S1>
@@ -1596,7 +1599,8 @@ static void launch_outer_inv(const DevPlan& d, void* ra, uint32_t nprod, hipStre
     const auto& set = SetOf<A>::get(d);
     const uint32_t wgs_per_row = ((d.N >> S1) + 255) / 256;
     const uint32_t grid = wgs_per_row * set.k * nprod;
-    ntt_outer_inv_kernel<A, S1><<<grid, 256, 0, s>>>(static_cast<T*>(ra), set.tw_inv, set.pc, d.log_n, set.k, nprod);
+    ntt_outer_inv_kernel<A, S1><<<grid, 256, 0, s>>>(static_cast<T*>(ra), set.tw_inv, set.pc, d.log_n, set.k, nprod,
+                                                      set.tw_log);
 }
 
 template <class A>
@@ -1659,7 +1663,7 @@ hipError_t launch_decompose(const DevPlan& d, int width, const uint32_t* a, void
         const DevSet30& w = d.w30;
         auto go = [&](auto kern) {
             kern<<<g, 256, 0, s>>>(a, ra, na, b, rb, w.mfma_tab, w.mfma_bias, w.pc, w.tw_fwd, d.N, d.L, w.k,
-                                   w.mfma_tiles, nb);
+                                   w.mfma_tiles, nb, w.tw_log);
         };
         switch (fuse_s1 * 3 + (ws == 1 ? 0 : ws == 2 ? 1 : 2)) {
             case 0: go(decompose30_mfma_kernel<0, 1>); break;
@@ -1726,7 +1730,7 @@ static void launch_crt_mfma(const DevPlan& d, const uint32_t* res, uint32_t* out
     if constexpr (KCT > 0)
         crt30_mfma_kernel<RT, KCT, LT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
                                                          d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, nullptr,
-                                                         nullptr);
+                                                         nullptr, 0u);
     else
         crt30_mfma_generic_kernel<RT><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc,
                                                             d.w30.qc, d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod);
@@ -1747,7 +1751,7 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     const uint32_t grid = groups < cap ? groups : cap;
     crt30_mfma_kernel<4, 8, 28, 2><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
                                                         d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, d.w30.pc,
-                                                        d.w30.tw_inv);
+                                                        d.w30.tw_inv, d.w30.tw_log);
     return hipGetLastError();
 }
 

@@ -12,8 +12,9 @@ namespace nc {
 // Resident tables of the 62-bit prime set (the reference's prime rule).
 struct DevSet62 {
     uint32_t k = 0;
-    const Tw* tw_fwd = nullptr;     // [k][N] psi^bitrev(x) for x in [1, N)
-    const Tw* tw_inv = nullptr;     // [k][N] psi^-bitrev(x)
+    uint32_t tw_log = 0;            // twiddle rows are 2^tw_log entries apart (log2 N, or the shared set's 16)
+    const Tw* tw_fwd = nullptr;     // [k][2^tw_log] psi^bitrev(x) for x in [1, N)
+    const Tw* tw_inv = nullptr;     // [k][2^tw_log] psi^-bitrev(x)
     const PrimeConst* pc = nullptr; // [k]
     const uint64_t* place = nullptr;// [k][kMaxLimbs] 2^(32 l) 2^64 mod p_j
     const CrtConst* cc = nullptr;   // [k]
@@ -24,6 +25,7 @@ struct DevSet62 {
 // Resident tables of the 30-bit prime set.
 struct DevSet30 {
     uint32_t k = 0;
+    uint32_t tw_log = 0;  // twiddle row stride, as in DevSet62
     const Tw32* tw_fwd = nullptr;
     const Tw32* tw_inv = nullptr;
     const PrimeConst32* pc = nullptr;

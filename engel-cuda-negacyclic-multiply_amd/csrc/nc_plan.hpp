@@ -58,6 +58,11 @@ struct HostTables {
 // message on failure.
 HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles = true);
 
+// Degree-N tables on a shared set whose master tables `m` were built (without
+// host twiddles) for its largest degree: same primes and N-independent
+// tables, the powers table and N-dependent per-prime constants for N.
+HostTables derive_tables(const HostTables& m, uint32_t N);
+
 // The stored twiddle entry x of prime j (direction inverse), by host
 // arithmetic: {psi^(+-bitrev(x)), Shoup quotient}, {0, 0} at x = 0, the
 // 30-bit forward value negated.  Checks samples of device-built tables.
@@ -81,8 +86,11 @@ struct Choice {
 
 }  // namespace nc
 
+struct nc_tableset;
+
 struct nc_plan {
     nc::HostTables host;
+    nc_tableset* shared = nullptr;  // the table set this plan's tables live in (nullptr: its own)
     nc::DevPlan dev{};
     int device = 0;
     // resident device tables (one allocation)

@@ -151,6 +151,7 @@ struct nc_server {
     bool ready = false;                       // under mu: lanes warmed (or failed)
     int warm_rc = NC_OK;                      // under mu
     nc_server_stats st{};                     // under mu
+    nc_tableset* tables = nullptr;            // one table set for every served degree
     Arena arena;                              // scheduler thread only (after creation)
     void* arena_mem = nullptr;
     std::thread th;
@@ -374,6 +375,7 @@ void nc_server_destroy(nc_server* sv) {
         sv->th.join();
     }
     for (auto& kv : sv->deg) free_degree(kv.second);  // every lane stream is drained first
+    nc_tableset_destroy(sv->tables);
     if (sv->arena_mem) (void)hipFree(sv->arena_mem);
     for (auto& kv : sv->reqs) delete kv.second;
     delete sv;
@@ -398,6 +400,7 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
     };
     hipError_t e = hipGetDevice(&sv->device);
     if (e != hipSuccess) return fail_out(hip_err(e, "hipGetDevice"));
+    if (int rc = nc_tableset_create(cfg->L, sv->q.data(), &sv->tables)) return fail_out(rc);
     uint64_t most = 0;  // every lane of every degree holding a full batch at once
     for (uint32_t i = 0; i < cfg->n_degrees; ++i) {
         const uint32_t N = cfg->degrees[i];
@@ -405,7 +408,7 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
         if ((uint64_t)cfg->max_batch * N > (1ull << 31)) return fail_out(set_error(NC_ERR_CAPACITY, "max_batch * N above 2^31"));
         Degree& d = sv->deg[N];
         d.N = N;
-        if (int rc = nc_plan_create(N, cfg->L, sv->q.data(), &d.plan)) return fail_out(rc);
+        if (int rc = nc_plan_create_shared(sv->tables, N, &d.plan)) return fail_out(rc);
         d.words = (uint64_t)N * cfg->L;
         d.ws = nc::workspace_bytes_per_product(d.plan);
         d.footprint = d.block_bytes(1);

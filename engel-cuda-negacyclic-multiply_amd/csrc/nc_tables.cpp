@@ -392,6 +392,55 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twi
     return t;
 }
 
+// Degree-N tables on a shared set (the master's tables, built for its
+// largest degree Nm with primes p = 1 mod 2 Nm): psi_N = psi_Nm^(Nm / N), so
+// psi_N^(2^i) is the master's psi^(2^(i + log Nm - log N)) -- the tail of its
+// powers table -- and degree N's twiddle entry x < N is the master's entry x
+// (bitrev over log Nm bits of x < N is bitrev over log N bits shifted up by
+// log Nm - log N).  Only the per-prime constants that carry N^-1 and
+// psi_N^(-N/2) are recomputed; every other table is N-independent.
+HostTables derive_tables(const HostTables& m, uint32_t N) {
+    if (N < 2 || N > m.N || (N & (N - 1))) throw std::invalid_argument("N must be a power of two from 2 to the set's largest degree");
+    if (m.host_twiddles) throw std::invalid_argument("a shared set keeps its twiddles on the device");
+    HostTables t = m;
+    t.N = N;
+    t.log_n = 0;
+    while ((1u << t.log_n) < N) ++t.log_n;
+    const uint32_t sh = m.log_n - t.log_n;
+    auto tail = [&](const auto& src, auto& dst, size_t k) {
+        dst.resize(k * 2 * t.log_n);
+        for (size_t j = 0; j < k; ++j)
+            for (size_t dir = 0; dir < 2; ++dir)
+                for (uint32_t i = 0; i < t.log_n; ++i)
+                    dst[(j * 2 + dir) * t.log_n + i] = src[(j * 2 + dir) * m.log_n + i + sh];
+    };
+    tail(m.s62.tw_pow, t.s62.tw_pow, m.s62.primes.size());
+    tail(m.s30.tw_pow, t.s30.tw_pow, m.s30.primes.size());
+    for (size_t j = 0; j < m.s62.primes.size(); ++j) {
+        const uint64_t p = m.s62.primes[j];
+        const uint64_t R = (uint64_t)((((u128)1) << 64) % p);
+        const uint64_t nr = mulmod(mulmod(invmod(N % p, p), R, p), m.s62.cc[j].y, p);
+        const uint64_t wr = mulmod(t.s62.tw_pow[(j * 2 + 1) * t.log_n + t.log_n - 1].w, nr, p);  // psi_N^-(N/2)
+        PrimeConst& c = t.s62.pc[j];
+        c.nr = nr;
+        c.nr_s = shoup_w(nr, p, 64);
+        c.wr = wr;
+        c.wr_s = shoup_w(wr, p, 64);
+    }
+    for (size_t j = 0; j < m.s30.primes.size(); ++j) {
+        const uint64_t p = m.s30.primes[j];
+        const uint64_t R = (1ull << 32) % p;
+        const uint64_t nr = mulmod(mulmod(invmod(N % p, p), R, p), m.s30.cc[j].y, p);
+        const uint64_t wr = mulmod(t.s30.tw_pow[(j * 2 + 1) * t.log_n + t.log_n - 1].w, nr, p);
+        PrimeConst32& c = t.s30.pc[j];
+        c.nr = (uint32_t)nr;
+        c.nr_s = (uint32_t)shoup_w(nr, p, 32);
+        c.wr = (uint32_t)wr;
+        c.wr_s = (uint32_t)shoup_w(wr, p, 32);
+    }
+    return t;
+}
+
 Tw twiddle_entry62(const HostTables& t, uint32_t j, uint32_t x, bool inverse) {
     if (x == 0) return Tw{0, 0};
     const uint64_t p = t.s62.primes[j];

@@ -88,6 +88,12 @@ def lib():
         L.nc_host_twiddles.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, ctypes.c_int, _vp,
                                        ctypes.POINTER(ctypes.c_size_t)]
         L.nc_plan_copy_twiddles.argtypes = [_vp, ctypes.c_uint32, ctypes.c_int, _vp, ctypes.c_size_t]
+        L.nc_tableset_create.argtypes = [ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)]
+        L.nc_tableset_destroy.argtypes = [_vp]
+        L.nc_tableset_destroy.restype = None
+        L.nc_tableset_get_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                           ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)]
+        L.nc_plan_create_shared.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(_vp)]
         L.nc_plan_destroy.argtypes = [_vp]
         L.nc_plan_destroy.restype = None
         L.nc_plan_get_info.argtypes = [_vp, ctypes.POINTER(PlanInfo)]
@@ -196,15 +202,54 @@ def _stream_handle(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
-class Plan:
-    """One (N, q) plan: resident prime tables on the current device."""
+class TableSet:
+    """One shared table set for every degree N <= 65536 of one modulus
+    (nc_tableset_create); plans on it come from Plan(N, q, tables=set)."""
 
-    def __init__(self, N: int, q: np.ndarray):
+    def __init__(self, q: np.ndarray):
+        self.q = np.ascontiguousarray(q, dtype=np.uint32)
+        h = _vp()
+        _check(lib().nc_tableset_create(self.q.shape[0], _p32(self.q), ctypes.byref(h)), "nc_tableset_create")
+        self._h = h
+
+    def info(self) -> dict:
+        k62, k30, nb, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_float()
+        _check(lib().nc_tableset_get_info(self._h, ctypes.byref(k62), ctypes.byref(k30), ctypes.byref(nb),
+                                          ctypes.byref(ms)), "nc_tableset_get_info")
+        return {"k62": k62.value, "k30": k30.value, "table_bytes": nb.value, "create_ms": ms.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nc_tableset_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Plan:
+    """One (N, q) plan: resident prime tables on the current device -- its
+    own (nc_plan_create), or a shared set's (tables=TableSet,
+    nc_plan_create_shared)."""
+
+    def __init__(self, N: int, q: np.ndarray, tables: TableSet = None):
         self.q = np.ascontiguousarray(q, dtype=np.uint32)
         self.N = int(N)
         self.L = int(self.q.shape[0])
         h = _vp()
-        _check(lib().nc_plan_create(self.N, self.L, _p32(self.q), ctypes.byref(h)), "nc_plan_create")
+        if tables is None:
+            _check(lib().nc_plan_create(self.N, self.L, _p32(self.q), ctypes.byref(h)), "nc_plan_create")
+        else:
+            _check(lib().nc_plan_create_shared(tables._h, self.N, ctypes.byref(h)), "nc_plan_create_shared")
         self._h = h
         self.info = self.get_info()
         self.k62, self.k30 = self.info.k62, self.info.k30

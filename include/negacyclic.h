@@ -92,6 +92,34 @@ void nc_plan_destroy(nc_plan* plan);
 int nc_plan_get_info(const nc_plan* plan, nc_plan_info* info);
 /* The primes of one set (width 62 or 30), host copy; out holds k values. */
 int nc_plan_primes(const nc_plan* plan, uint32_t width, uint64_t* out);
+/* ---- shared table sets (no counterpart in the reference) ---------------
+ * One set of resident tables for every degree N <= 65536 of one modulus q
+ * (L limbs): both prime sets chosen once by the prime rule at N = 65536
+ * (p = 1 mod 2^17, so every smaller degree's NTTs exist in them), the
+ * 65536-entry twiddle rows of every prime built on the device, the CRT and
+ * matrix-core tables built once.  A plan of degree N on the set
+ * (nc_plan_create_shared) uses psi_N = psi_65536^(65536 / N), whose twiddle
+ * table is the first N entries of every row, and owns only its N-dependent
+ * per-prime constants and its staging: the table work the reference does in
+ * fherma_init (solve.cu:1150-1487) is paid once per modulus instead of once
+ * per degree, and one set serves every degree.  Products are bit-identical
+ * to those of nc_plan_create's plans; the primes differ from the per-N rule
+ * (solve.cu:1179-1204) below N = 65536, so stage-level residues of a shared
+ * plan are over the set's primes (nc_plan_primes). */
+typedef struct nc_tableset nc_tableset;
+int nc_tableset_create(uint32_t L, const uint32_t* q, nc_tableset** out);
+/* Drops the creator's reference (null safe); the set is freed once every
+ * plan created from it has been destroyed as well. */
+void nc_tableset_destroy(nc_tableset* set);
+/* Primes per set, resident device bytes, and the wall time nc_tableset_create
+ * took (any out pointer may be NULL). */
+int nc_tableset_get_info(const nc_tableset* set, uint32_t* k62, uint32_t* k30, uint64_t* table_bytes,
+                         float* create_ms);
+/* A plan of degree N (power of two in [2, 65536]) on the set's tables, on
+ * the set's device (the current device must be that one).  Its
+ * nc_plan_info.table_bytes counts only its own per-prime constants. */
+int nc_plan_create_shared(nc_tableset* set, uint32_t N, nc_plan** out);
+
 /* Copies one device twiddle table of a plan to host memory: width 62 gives
  * k x N {w, ws} pairs of uint64 (16 bytes each), width 30 k x N pairs of
  * uint32 (8 bytes, forward w stored as 2^32 - w); inverse selects psi^-1.

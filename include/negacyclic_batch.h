@@ -32,7 +32,8 @@ typedef struct nc_group {
 typedef struct nc_batch nc_batch;
 typedef struct nc_graph nc_graph;
 
-/* Resident per-degree plans for one modulus q (L limbs), created on first use. */
+/* Per-degree plans for one modulus q (L limbs), created on first use, all on
+ * one table set (nc_tableset_create) made with the first plan. */
 int nc_batch_create(uint32_t L, const uint32_t* q, nc_batch** out);
 /* Destroys the plans; every graph captured from this batch must be gone. */
 void nc_batch_destroy(nc_batch* batch);
@@ -71,7 +72,8 @@ void nc_graph_destroy(nc_graph* graph);
 typedef struct nc_server_config {
     uint32_t L;
     const uint32_t* q;
-    const uint32_t* degrees;   /* ring degrees served (tables resident)    */
+    const uint32_t* degrees;   /* ring degrees served (one resident table
+                                  set for all of them, nc_tableset_create) */
     uint32_t n_degrees;
     uint32_t max_batch;        /* products per launch (per lane)            */
     uint32_t lanes_per_degree; /* batches of one degree in flight at once   */

@@ -507,3 +507,95 @@ print(json.dumps({"info": info, "choice": list(choice), "exact": exact}))
     say("second process", got)
     assert got["info"]["from_cache"] and got["info"]["tune_ms"] < 50.0
     assert tuple(got["choice"]) == choice and got["exact"]
+
+
+# ---- shared table sets (nc_tableset_create / nc_plan_create_shared) -------
+
+_SETS = {}
+
+
+def _table_set(name):
+    if name not in _SETS:
+        _SETS[name] = ncmul.TableSet(modulus(name))
+    return _SETS[name]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c['modulus']}-N{c['N']}-s{c['seed']}")
+def test_golden_shared_tables(gpu, case):
+    """Every golden vector through a plan on its modulus's one shared table
+    set (primes p = 1 mod 2^17, degree N reading the prefix of 65536-entry
+    twiddle rows), for both prime widths."""
+    q = modulus(case["modulus"])
+    N, s = case["N"], case["seed"]
+    ts = _table_set(case["modulus"])
+    a, b = ncmul.gen_uniform(2 * s, N, q), ncmul.gen_uniform(2 * s + 1, N, q)
+    with ncmul.Plan(N, q, tables=ts) as plan:
+        for width in WIDTHS:
+            say("golden shared", case["modulus"], N, s, "width", width)
+            plan.set_tuning(width)
+            c = plan.multiply(a, b)
+            assert sha(c) == case["sha256_c"], width
+            for idx, hx in case["samples"].items():
+                assert O.limbs_to_int(c[int(idx)]) == int(hx, 16)
+
+
+def test_shared_tables_are_prefixes_of_the_master(gpu, q868):
+    """The set's 65536-entry rows equal the host construction at N = 65536
+    (the prime rule and psi rule there are the per-N ones), and every degree's
+    table is the prefix of every row; a shared plan's primes are the set's."""
+    ts = _table_set("q868")
+    info = ts.info()
+    say("table set", info)
+    with ncmul.Plan(65536, q868, tables=ts) as top:
+        full = {(w, inv): top.twiddles(w, inv) for w in WIDTHS for inv in (False, True)}
+        for (w, inv), t in full.items():
+            assert np.array_equal(t, ncmul.host_twiddles(65536, q868, w, inv)), (w, inv)
+        for N in (2, 16, 1024, 16384):
+            with ncmul.Plan(N, q868, tables=ts) as p:
+                assert np.array_equal(p.primes(30), top.primes(30)) and np.array_equal(p.primes(62), top.primes(62))
+                assert all(int(x) % (1 << 17) == 1 for x in p.primes(30))
+                for (w, inv), t in full.items():
+                    assert np.array_equal(p.twiddles(w, inv), t[:, :N]), (N, w, inv)
+
+
+@pytest.mark.parametrize("mfma", [3, 0])
+def test_shared_plan_stage_values(gpu, q868, mfma):
+    """Stage level on a shared plan: CRT-weighted product residues over the
+    set's primes equal exact Python integers, and the CRT maps them back."""
+    N = 2048
+    qi = O.limbs_to_int(q868)
+    a, b = ncmul.gen_uniform(35, N, q868), ncmul.gen_uniform(36, N, q868)
+    cint = O.py_negacyclic_exact(O.poly_to_ints(a), O.poly_to_ints(b))
+    with ncmul.Plan(N, q868, tables=_table_set("q868")) as plan:
+        plan.set_mfma(mfma)
+        pr = [int(p) for p in plan.primes(30)]
+        y = crt_weights(pr)
+        want = np.array([[c % p * y[j] % p for c in cint] for j, p in enumerate(pr)], dtype=np.uint32)
+        d_a, d_b = ncmul.to_device_u32(a[None]), ncmul.to_device_u32(b[None])
+        d_res = ncmul.empty_device_u32((1, plan.k30, N))
+        plan.stage_product_residues(30, d_a, d_b, d_res, 1)
+        assert np.array_equal(ncmul.from_device_u32(d_res)[0], want)
+        d_c = ncmul.empty_device_u32((1, N, 28))
+        plan.stage_crt(30, ncmul.to_device_u32(want[None]), d_c, 1)
+        assert O.poly_to_ints(ncmul.from_device_u32(d_c)[0]) == [c % qi for c in cint]
+
+
+@pytest.mark.parametrize("N", [64, 2048, 16384, 65536])
+def test_shared_plan_every_split(gpu, q868, N):
+    """Every accepted split and width on a shared plan, batched, equals the oracle."""
+    a = np.stack([ncmul.gen_uniform(51 + 2 * i, N, q868) for i in range(3)])
+    b = np.stack([ncmul.gen_uniform(52 + 2 * i, N, q868) for i in range(3)])
+    op = O.Plan(N, q868)
+    want = [op.multiply(a[i], b[i]) for i in range(3)]
+    logn = N.bit_length() - 1
+    with ncmul.Plan(N, q868, tables=_table_set("q868")) as plan:
+        for width in WIDTHS:
+            for s1 in range(max(0, logn - 14), min(4, logn - 4) + 1):
+                plan.set_tuning(width, s1)
+                d_c = ncmul.empty_device_u32((3, N, 28))
+                d_c.fill_(-1)
+                plan.multiply_device(ncmul.to_device_u32(a), ncmul.to_device_u32(b), d_c, 3)
+                got = ncmul.from_device_u32(d_c)
+                say("shared split", N, width, s1)
+                for i in range(3):
+                    assert np.array_equal(got[i], want[i]), (width, s1, i)
