@@ -443,6 +443,19 @@ def main() -> int:
             ncmul.Plan(n_create, q).close()
             ts.append((time.perf_counter() - t0) * 1e3)
         create_ms[str(n_create)] = min(ts)
+    # one shared table set for every degree (prime search, psi and CRT tables
+    # on the device), then each served degree's plan on it
+    t0 = time.perf_counter()
+    tset = ncmul.TableSet(q)
+    set_ms = (time.perf_counter() - t0) * 1e3
+    deg_ms = {}
+    for n_create in (1 << 11, 1 << 14, 1 << 16):
+        t0 = time.perf_counter()
+        ncmul.Plan(n_create, q, tables=tset).close()
+        deg_ms[str(n_create)] = (time.perf_counter() - t0) * 1e3
+    create_ms["shared_table_set"] = {"create_ms": set_ms, "table_bytes": tset.info()["table_bytes"],
+                                     "degree_plan_ms": deg_ms}
+    tset.close()
     batched = ncmul.Plan(N, q)
     plain = ncmul.Plan(N, q)
     tuning = None

@@ -551,6 +551,98 @@ int finish_plan(nc_plan* p) {
     return grow_workspace(p, 1);
 }
 
+// The device half of a table set's initialisation (the host does it in
+// fherma_init, solve.cu:1174-1204, 1342-1369): for each prime width, a
+// Miller-Rabin search over the candidates p = 1 mod 2N below the word bound
+// with each prime's psi, the prime rule applied to the results on the host,
+// then the big-integer CRT products mod q of the chosen primes and the
+// 30-bit set's int8 MFMA CRT table.  build_tables checks all of it.
+int device_init(uint32_t L, const uint32_t* q, uint32_t log_n, nc::DeviceInit& init) {
+    uint32_t q_bits = 0;
+    for (int l = (int)L - 1; l >= 0 && !q_bits; --l)
+        if (q[l]) q_bits = 32 * (uint32_t)l + 32 - (uint32_t)__builtin_clz(q[l]);
+    const uint64_t step = 2ull << log_n;
+    constexpr uint32_t kCands = 8192;
+    std::vector<void*> bufs;
+    auto dev_alloc = [&](size_t bytes, void** out) {
+        const hipError_t e = hipMalloc(out, bytes);
+        if (e == hipSuccess) bufs.push_back(*out);
+        return e;
+    };
+    auto run = [&]() -> int {
+        uint64_t* d_cand = nullptr;
+        NC_HIP(dev_alloc(kCands * sizeof(uint64_t), reinterpret_cast<void**>(&d_cand)), "prime search buffer");
+        for (int bits : {62, 30}) {
+            uint64_t top = (1ull << bits) - 1;
+            top -= (top - 1) % step;
+            const uint32_t count = (uint32_t)std::min<uint64_t>(kCands, (top - 1) / step);
+            NC_HIP(nc::launch_prime_search(top, step, count, d_cand, nullptr), "prime search");
+            std::vector<uint64_t> psi(count);
+            NC_HIP(hipMemcpy(psi.data(), d_cand, count * sizeof(uint64_t), hipMemcpyDeviceToHost), "prime search");
+            std::vector<std::pair<uint64_t, uint64_t>> cands(count);
+            for (uint32_t i = 0; i < count; ++i) cands[i] = {top - (uint64_t)i * step, psi[i]};
+            const bool ok = nc::select_primes(cands, log_n, q_bits, bits == 62 ? (size_t)nc::kMaxPrimes
+                                                                              : (size_t)nc::kMaxPrimes30,
+                                              bits == 62 ? init.primes62 : init.primes30,
+                                              bits == 62 ? init.psi62 : init.psi30);
+            if (!ok) return fail(NC_ERR_PRIMES, "not enough NTT primes for this modulus");
+        }
+        uint32_t* d_q = nullptr;
+        NC_HIP(dev_alloc(L * sizeof(uint32_t), reinterpret_cast<void**>(&d_q)), "modulus");
+        NC_HIP(hipMemcpy(d_q, q, L * sizeof(uint32_t), hipMemcpyHostToDevice), "modulus");
+        for (int bits : {62, 30}) {
+            const std::vector<uint64_t>& pr = bits == 62 ? init.primes62 : init.primes30;
+            std::vector<uint32_t>& prod = bits == 62 ? init.prod62 : init.prod30;
+            const uint32_t k = (uint32_t)pr.size();
+            uint64_t* d_p = nullptr;
+            uint32_t* d_prod = nullptr;
+            NC_HIP(dev_alloc(k * sizeof(uint64_t), reinterpret_cast<void**>(&d_p)), "primes");
+            NC_HIP(dev_alloc((k + 1) * nc::kMaxLimbs * sizeof(uint32_t), reinterpret_cast<void**>(&d_prod)),
+                   "CRT products");
+            NC_HIP(hipMemcpy(d_p, pr.data(), k * sizeof(uint64_t), hipMemcpyHostToDevice), "primes");
+            NC_HIP(nc::launch_crt_products(d_p, k, d_q, L, d_prod, nullptr), "CRT products");
+            prod.resize((size_t)(k + 1) * nc::kMaxLimbs);
+            NC_HIP(hipMemcpy(prod.data(), d_prod, prod.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                   "CRT products");
+            if (bits != 30 || q_bits + 54 < 32 * L) continue;  // no MFMA CRT for this shape
+            // (-M) mod q from M mod q, then the int8 table over (M / p_j) mod q
+            std::vector<uint32_t> neg_m(nc::kMaxLimbs, 0);
+            bool zero = true;
+            for (uint32_t l = 0; l < L; ++l) zero = zero && prod[(size_t)k * nc::kMaxLimbs + l] == 0;
+            uint32_t br = 0;
+            for (uint32_t l = 0; l < L && !zero; ++l) {
+                const uint64_t d = (uint64_t)q[l] - prod[(size_t)k * nc::kMaxLimbs + l] - br;
+                neg_m[l] = (uint32_t)d;
+                br = (uint32_t)(d >> 63);
+            }
+            const uint32_t rt = (L + 4 + 7) / 8, kc = (k + 7) / 8;
+            int8_t* d_tab = nullptr;
+            int64_t* d_corr = nullptr;
+            uint32_t *d_neg = nullptr, *d_negr = nullptr;
+            const size_t tab_bytes = (size_t)rt * 32 * 32 * kc;
+            NC_HIP(dev_alloc(tab_bytes, reinterpret_cast<void**>(&d_tab)), "CRT table");
+            NC_HIP(dev_alloc(8 * rt * sizeof(int64_t), reinterpret_cast<void**>(&d_corr)), "CRT table");
+            NC_HIP(dev_alloc(nc::kMaxLimbs * sizeof(uint32_t), reinterpret_cast<void**>(&d_neg)), "CRT table");
+            NC_HIP(dev_alloc(nc::kMaxLimbs * sizeof(uint32_t), reinterpret_cast<void**>(&d_negr)), "CRT table");
+            NC_HIP(hipMemset(d_tab, 0, tab_bytes), "CRT table");
+            NC_HIP(hipMemcpy(d_neg, neg_m.data(), nc::kMaxLimbs * sizeof(uint32_t), hipMemcpyHostToDevice), "CRT table");
+            NC_HIP(nc::launch_crt_mfma_tables(d_prod, d_p, k, d_q, L, rt, kc, d_neg, d_tab, d_corr, d_negr, nullptr),
+                   "CRT table");
+            init.crt_tab.resize(tab_bytes);
+            init.crt_c.resize(8 * rt);
+            init.neg_m_r.resize(nc::kMaxLimbs);
+            NC_HIP(hipMemcpy(init.crt_tab.data(), d_tab, tab_bytes, hipMemcpyDeviceToHost), "CRT table");
+            NC_HIP(hipMemcpy(init.crt_c.data(), d_corr, 8 * rt * sizeof(int64_t), hipMemcpyDeviceToHost), "CRT table");
+            NC_HIP(hipMemcpy(init.neg_m_r.data(), d_negr, nc::kMaxLimbs * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                   "CRT table");
+        }
+        return NC_OK;
+    };
+    const int rc = run();
+    for (void* b : bufs) (void)hipFree(b);
+    return rc;
+}
+
 // Runs a host-table step, mapping its exceptions to status codes.
 extern "C++" template <class F>
 int guarded_tables(F&& f) {
@@ -607,13 +699,19 @@ int nc_tableset_create(uint32_t L, const uint32_t* q, nc_tableset** out) {
     const auto t0 = std::chrono::steady_clock::now();
     nc_tableset* ts = new (std::nothrow) nc_tableset();
     if (!ts) return fail(NC_ERR_NOMEM, "table set allocation");
-    int rc = guarded_tables([&] {
-        ts->host = nc::build_tables(1u << nc::kMaxLogN, L, q, /*host_twiddles=*/false);
-        nc::self_test(ts->host);
-    });
+    int rc = guarded_tables([&] { nc::validate_modulus(L, q); });  // before any device work
     hipError_t e = hipSuccess;
     if (rc == NC_OK) e = hipGetDevice(&ts->device);
     if (rc == NC_OK && e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
+    // prime search, psi and the big-integer CRT tables on the device; the host
+    // checks them, builds the word-level constants and runs the self tests
+    nc::DeviceInit init;
+    if (rc == NC_OK) rc = device_init(L, q, nc::kMaxLogN, init);
+    if (rc == NC_OK)
+        rc = guarded_tables([&] {
+            ts->host = nc::build_tables(1u << nc::kMaxLogN, L, q, /*host_twiddles=*/false, &init);
+            nc::self_test(ts->host);
+        });
     if (rc == NC_OK) rc = upload_tables(ts->host, ts->device, &ts->d_tables, &ts->table_bytes, ts->dev);
     if (rc) {
         tableset_release(ts);

@@ -1795,6 +1795,253 @@ hipError_t launch_crt(const DevPlan& d, int width, const void* res, uint32_t* ou
 }
 
 // ===========================================================================
+// Table-set initialisation on the device (the work fherma_init does on the
+// host, solve.cu:1174-1204 and 1342-1369): the prime search with each
+// prime's psi, and the big-integer CRT tables mod q.
+// ===========================================================================
+
+// Montgomery arithmetic mod an odd n < 2^63 (R = 2^64), for the prime search.
+struct Mont64 {
+    uint64_t n, ninv;  // ninv = -n^-1 mod 2^64
+    __device__ explicit Mont64(uint64_t n_) : n(n_) {
+        uint64_t x = n_;  // Newton: x = n^-1 mod 2^(2^i), 6 steps from 3 bits
+        for (int i = 0; i < 6; ++i) x *= 2 - n_ * x;
+        ninv = 0 - x;
+    }
+    __device__ uint64_t mul(uint64_t a, uint64_t b) const {  // a b R^-1 mod n, a, b < n
+        const uint64_t lo = a * b, hi = __umul64hi(a, b);
+        const uint64_t m = lo * ninv;
+        const uint64_t mlo = m * n, mhi = __umul64hi(m, n);
+        const uint64_t c = (lo + mlo) < lo ? 1u : 0u;  // carry of the low words (their sum is 0 mod 2^64)
+        const uint64_t r = hi + mhi + c;               // < 2n
+        return r >= n ? r - n : r;
+    }
+    __device__ uint64_t to(uint64_t a) const {  // a R mod n
+        return (uint64_t)(((unsigned __int128)(a % n) << 64) % n);
+    }
+    __device__ uint64_t pow(uint64_t a_m, uint64_t e) const {  // a_m in Montgomery form; result too
+        uint64_t r = to(1), b = a_m;
+        for (; e; e >>= 1) {
+            if (e & 1) r = mul(r, b);
+            b = mul(b, b);
+        }
+        return r;
+    }
+};
+
+// Deterministic Miller-Rabin for n < 3.3e24 (the first twelve prime bases).
+__device__ bool mr_prime(uint64_t n) {
+    if (n < 2) return false;
+    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (uint32_t b : bases)
+        if (n % b == 0) return n == b;
+    const Mont64 M(n);
+    uint64_t d = n - 1;
+    int s = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        ++s;
+    }
+    const uint64_t one = M.to(1), minus_one = M.to(n - 1);
+    for (uint32_t b : bases) {
+        uint64_t x = M.pow(M.to(b), d);
+        if (x == one || x == minus_one) continue;
+        bool witness = true;
+        for (int r = 1; r < s && witness; ++r) {
+            x = M.mul(x, x);
+            if (x == minus_one) witness = false;
+        }
+        if (witness) return false;
+    }
+    return true;
+}
+
+// Candidate i is top - i step (top = 1 mod step).  out[i] = 0 if it is not
+// prime, else psi = g^((p - 1) / step) for the smallest g >= 2 with
+// g^((p - 1) / 2) = -1 (a primitive step-th root of unity: the same psi rule
+// as the host's, solve.cu:495-502).
+__global__ __launch_bounds__(64) void prime_search_kernel(uint64_t top, uint64_t step, uint32_t count,
+                                                          uint64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= count) return;
+    const uint64_t p = top - (uint64_t)i * step;
+    uint64_t psi = 0;
+    if (p > step && mr_prime(p)) {
+        const Mont64 M(p);
+        const uint64_t minus_one = M.to(p - 1);
+        for (uint64_t g = 2; g < 100000; ++g) {
+            const uint64_t gm = M.to(g);
+            if (M.pow(gm, (p - 1) / 2) == minus_one) {
+                psi = M.mul(M.pow(gm, (p - 1) / step), 1);  // out of Montgomery form
+                break;
+            }
+        }
+    }
+    out[i] = psi;
+}
+
+// ---- L-limb arithmetic mod q (x < q throughout), one big number per thread
+__device__ bool limbs_geq(const uint32_t* x, const uint32_t* q, uint32_t L) {
+    for (int l = (int)L - 1; l >= 0; --l)
+        if (x[l] != q[l]) return x[l] > q[l];
+    return true;
+}
+
+__device__ void limbs_sub(uint32_t* x, const uint32_t* q, uint32_t L) {
+    uint32_t br = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        uint32_t nb;
+        x[l] = __builtin_subc(x[l], q[l], br, &nb);
+        br = nb;
+    }
+}
+
+__device__ void dbl_mod(uint32_t* x, const uint32_t* q, uint32_t L) {  // x = 2 x mod q
+    uint32_t top = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        const uint32_t nt = x[l] >> 31;
+        x[l] = (x[l] << 1) | top;
+        top = nt;
+    }
+    if (top || limbs_geq(x, q, L)) limbs_sub(x, q, L);  // 2x < 2q
+}
+
+__device__ void add_mod(uint32_t* x, const uint32_t* y, const uint32_t* q, uint32_t L) {  // x = x + y mod q
+    uint32_t c = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        uint32_t nc;
+        x[l] = __builtin_addc(x[l], y[l], c, &nc);
+        c = nc;
+    }
+    if (c || limbs_geq(x, q, L)) limbs_sub(x, q, L);
+}
+
+__device__ void mul_word_mod(uint32_t* x, uint64_t w, const uint32_t* q, uint32_t L) {  // x = x w mod q
+    uint32_t acc[kMaxLimbs];
+    for (uint32_t l = 0; l < L; ++l) acc[l] = 0;
+    for (int bit = 63; bit >= 0; --bit) {
+        dbl_mod(acc, q, L);
+        if ((w >> bit) & 1u) add_mod(acc, x, q, L);
+    }
+    for (uint32_t l = 0; l < L; ++l) x[l] = acc[l];
+}
+
+// Thread j < k: (M / p_j) mod q = prod_{i != j} p_i mod q; thread k: M mod q.
+// out: [k + 1][kMaxLimbs].
+__global__ __launch_bounds__(64) void crt_products_kernel(const uint64_t* __restrict__ primes, uint32_t k,
+                                                          const uint32_t* __restrict__ q, uint32_t L,
+                                                          uint32_t* __restrict__ out) {
+    const uint32_t j = blockIdx.x * 64 + threadIdx.x;
+    if (j > k) return;
+    uint32_t qq[kMaxLimbs], x[kMaxLimbs];
+    for (uint32_t l = 0; l < L; ++l) {
+        qq[l] = q[l];
+        x[l] = l == 0 ? 1u : 0u;
+    }
+    for (uint32_t i = 0; i < k; ++i)
+        if (i != j) mul_word_mod(x, primes[i], qq, L);
+    for (uint32_t l = 0; l < (uint32_t)kMaxLimbs; ++l) out[(size_t)j * kMaxLimbs + l] = l < L ? x[l] : 0u;
+}
+
+// One column kappa = 4 j + u of the int8 MFMA CRT table (layout, offsets and
+// meaning as in nc_tables.cpp build_crt_mfma): q-limb rows hold the bytes of
+// m_ju = 2^(8u) ((M / p_j) mod q) 2^96 mod q, fraction rows the bytes of
+// 2^(8u) floor(2^56 / p_j), ones rows 1 at byte 0; raw bytes are stored
+// offset by -128; columns of j >= k hold raw zeros.  miq: [k][kMaxLimbs].
+__device__ __forceinline__ uint32_t crt_row_of(uint32_t lam, uint32_t b) {
+    return 32 * (lam / 8) + 8 * ((lam % 8) / 2) + 4 * (lam % 2) + b;
+}
+
+__global__ __launch_bounds__(64) void crt_mfma_table_kernel(const uint32_t* __restrict__ miq,
+                                                            const uint64_t* __restrict__ primes, uint32_t k,
+                                                            const uint32_t* __restrict__ q, uint32_t L, uint32_t rt,
+                                                            uint32_t kc, int8_t* __restrict__ tab) {
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= 32 * kc) return;
+    const uint32_t j = t / 4, u = t % 4, kappa = t, kbytes = 32 * kc;
+    const uint32_t lf = 8 * rt - 4, lq = lf < (uint32_t)kMaxLimbs ? lf : (uint32_t)kMaxLimbs;
+    uint32_t qq[kMaxLimbs], m[kMaxLimbs];
+    uint64_t g = 0;
+    for (uint32_t l = 0; l < L; ++l) {
+        qq[l] = q[l];
+        m[l] = j < k ? miq[(size_t)j * kMaxLimbs + l] : 0u;
+    }
+    if (j < k) {
+        for (uint32_t d = 0; d < 96 + 8 * u; ++d) dbl_mod(m, qq, L);
+        g = ((uint64_t)1 << 56) / primes[j];
+    }
+    for (uint32_t lam = 0; lam < lq; ++lam)
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t raw = lam < L ? (m[lam] >> (8 * b)) & 0xffu : 0u;
+            tab[(size_t)crt_row_of(lam, b) * kbytes + kappa] = (int8_t)((int)raw - 128);
+        }
+    const uint64_t gu = g << (8 * u);
+    for (uint32_t e = 0; e < 2; ++e)
+        for (uint32_t b = 0; b < 4; ++b)
+            tab[(size_t)crt_row_of(lf + e, b) * kbytes + kappa] =
+                (int8_t)((int)((gu >> (8 * (4 * e + b))) & 0xffu) - 128);
+    for (uint32_t e = 2; e < 4; ++e) tab[(size_t)crt_row_of(lf + e, 0) * kbytes + kappa] = 1;
+}
+
+// Per pseudo-limb corrections sum_b 2^(8b) (128 SA - 128^2 Kdim), SA the
+// raw-byte row sums (q-limb and fraction rows), and, in thread 8 rt, the
+// Montgomery-finish constant ((-M) mod q) 2^96 mod q from neg_m.
+__global__ __launch_bounds__(64) void crt_mfma_corr_kernel(const int8_t* __restrict__ tab, uint32_t rt, uint32_t kc,
+                                                           const uint32_t* __restrict__ neg_m,
+                                                           const uint32_t* __restrict__ q, uint32_t L,
+                                                           int64_t* __restrict__ corr, uint32_t* __restrict__ neg_m_r) {
+    const uint32_t lam = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t lf = 8 * rt - 4, lq = lf < (uint32_t)kMaxLimbs ? lf : (uint32_t)kMaxLimbs;
+    const uint32_t kbytes = 32 * kc;
+    if (lam < 8 * rt) {
+        int64_t c = 0;
+        if (lam < lq || lam == lf || lam == lf + 1) {
+            const int64_t off = 128ll * 128ll * kbytes;
+            for (uint32_t b = 0; b < 4; ++b) {
+                int64_t sa = 0;
+                for (uint32_t kappa = 0; kappa < kbytes; ++kappa) sa += (int)tab[(size_t)crt_row_of(lam, b) * kbytes + kappa] + 128;
+                c += (128 * sa - off) * ((int64_t)1 << (8 * b));
+            }
+        }
+        corr[lam] = c;
+    } else if (lam == 8 * rt) {
+        uint32_t qq[kMaxLimbs], x[kMaxLimbs];
+        for (uint32_t l = 0; l < L; ++l) {
+            qq[l] = q[l];
+            x[l] = neg_m[l];
+        }
+        for (uint32_t d = 0; d < 96; ++d) dbl_mod(x, qq, L);
+        for (uint32_t l = 0; l < (uint32_t)kMaxLimbs; ++l) neg_m_r[l] = l < L ? x[l] : 0u;
+    }
+}
+
+hipError_t launch_prime_search(uint64_t top, uint64_t step, uint32_t count, uint64_t* out, hipStream_t s) {
+    if (count == 0 || step == 0 || top < step) return hipErrorInvalidValue;
+    prime_search_kernel<<<(count + 63) / 64, 64, 0, s>>>(top, step, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_crt_products(const uint64_t* primes, uint32_t k, const uint32_t* q, uint32_t L, uint32_t* out,
+                               hipStream_t s) {
+    if (k == 0 || L == 0 || L > (uint32_t)kMaxLimbs) return hipErrorInvalidValue;
+    crt_products_kernel<<<(k + 1 + 63) / 64, 64, 0, s>>>(primes, k, q, L, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_crt_mfma_tables(const uint32_t* miq, const uint64_t* primes, uint32_t k, const uint32_t* q,
+                                  uint32_t L, uint32_t rt, uint32_t kc, const uint32_t* neg_m, int8_t* tab,
+                                  int64_t* corr, uint32_t* neg_m_r, hipStream_t s) {
+    if (k == 0 || L == 0 || L > (uint32_t)kMaxLimbs || rt == 0 || rt > (uint32_t)kCrtMaxRT || kc == 0 ||
+        kc > (uint32_t)kCrtMaxKC || k > 8 * kc)
+        return hipErrorInvalidValue;
+    crt_mfma_table_kernel<<<(32 * kc + 63) / 64, 64, 0, s>>>(miq, primes, k, q, L, rt, kc, tab);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    crt_mfma_corr_kernel<<<(8 * rt + 1 + 63) / 64, 64, 0, s>>>(tab, rt, kc, neg_m, q, L, corr, neg_m_r);
+    return hipGetLastError();
+}
+
+// ===========================================================================
 // Scratch of nc_multiply_device_noalloc: the residue rows of a and b of an
 // allocation-free call.  Each device holds its own copy for the life of the
 // process; host code uses the address hipGetSymbolAddress returns for the

@@ -106,6 +106,20 @@ hipError_t launch_twiddles30(Tw32* fwd, Tw32* inv, const Tw32* pow, const PrimeC
 bool crt_fuses_outer_inv(const DevPlan& d, int width, bool mfma, uint32_t log_blocks);
 hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out, uint32_t nprod, hipStream_t s);
 
+// Table-set initialisation on the device.
+// Candidates top - i step, i < count (top = 1 mod step): out[i] = 0 if not
+// prime, else the primitive step-th root psi = g^((p - 1) / step) of the
+// smallest quadratic non-residue g.
+hipError_t launch_prime_search(uint64_t top, uint64_t step, uint32_t count, uint64_t* out, hipStream_t s);
+// out [k + 1][kMaxLimbs]: row j < k (M / p_j) mod q, row k M mod q (primes[k], q[L]).
+hipError_t launch_crt_products(const uint64_t* primes, uint32_t k, const uint32_t* q, uint32_t L, uint32_t* out,
+                               hipStream_t s);
+// The int8 MFMA CRT table [rt * 32][kc * 32], its corrections [8 rt] and
+// ((-M) mod q) 2^96 mod q [kMaxLimbs] from miq [k][kMaxLimbs] and neg_m.
+hipError_t launch_crt_mfma_tables(const uint32_t* miq, const uint64_t* primes, uint32_t k, const uint32_t* q,
+                                  uint32_t L, uint32_t rt, uint32_t kc, const uint32_t* neg_m, int8_t* tab,
+                                  int64_t* corr, uint32_t* neg_m_r, hipStream_t s);
+
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 

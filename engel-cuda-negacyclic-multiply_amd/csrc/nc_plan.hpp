@@ -53,10 +53,36 @@ struct HostTables {
     HostSet30 s30;  // 30-bit primes: the same rule below 2^30
 };
 
+// Pieces of a table build made on the device (nc_tableset_create): the
+// chosen primes of each set (descending) with their psi_N, the big-integer
+// CRT products mod q, and the 30-bit set's int8 MFMA CRT table.  The builder
+// checks every prime (Miller-Rabin, p = 1 mod 2N, the word bound, the
+// coefficient bound) and recomputes sampled products with host arithmetic;
+// the init self tests check psi and emulate the MFMA CRT over the table.
+struct DeviceInit {
+    std::vector<uint64_t> primes62, psi62, primes30, psi30;
+    std::vector<uint32_t> prod62, prod30;  // [k + 1][kMaxLimbs]: (M / p_j) mod q for j < k, then M mod q
+    std::vector<int8_t> crt_tab;           // empty: build it on the host
+    std::vector<int64_t> crt_c;
+    std::vector<uint32_t> neg_m_r;         // [kMaxLimbs]
+};
+
 // Builds both prime sets and every table (the [k][N] twiddles only when
-// host_twiddles); throws std::invalid_argument / std::runtime_error with a
-// message on failure.
-HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles = true);
+// host_twiddles), from `init`'s device-made pieces where given; throws
+// std::invalid_argument / std::runtime_error with a message on failure.
+HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles = true,
+                        const DeviceInit* init = nullptr);
+
+// Throws std::invalid_argument unless q is an odd modulus of L <= 32 limbs
+// with 2 <= bits(q) and bits(q) + 70 <= 1088.
+void validate_modulus(uint32_t L, const uint32_t* q);
+
+// The prime rule over device-searched candidates: cands[i] = (p, psi or 0
+// when p is not prime), descending; takes primes while bits(prod p) < log2 N
+// + 2 bits(q) + 4 (as pick_primes does).  Returns false if the candidates
+// run out first.
+bool select_primes(const std::vector<std::pair<uint64_t, uint64_t>>& cands, uint32_t log_n, uint32_t q_bits,
+                   size_t max_k, std::vector<uint64_t>& primes, std::vector<uint64_t>& psi);
 
 // Degree-N tables on a shared set whose master tables `m` were built (without
 // host twiddles) for its largest degree: same primes and N-independent

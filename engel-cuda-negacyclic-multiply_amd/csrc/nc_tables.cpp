@@ -63,8 +63,8 @@ struct RootTables {
     uint64_t nr = 0, wr = 0;         // N^-1 R mod p, psi^-bitrev(1) N^-1 R mod p
 };
 
-RootTables roots(uint64_t p, uint32_t N, uint32_t log_n, int word_bits, bool full) {
-    const uint64_t psi = find_psi(p, N);
+RootTables roots(uint64_t p, uint32_t N, uint32_t log_n, int word_bits, bool full, uint64_t psi_given = 0) {
+    const uint64_t psi = psi_given ? psi_given : find_psi(p, N);
     const uint64_t psi_inv = invmod(psi, p);
     RootTables r;
     r.pf.resize(log_n);
@@ -112,27 +112,50 @@ uint64_t neg_inv(uint64_t p, int word_bits) {
 
 // CRT constants common to both widths: miq = (M/p_j) mod q, y_j, and the
 // QConst (q, (-M) mod q, Barrett constant).
+// (M / p_j) mod q by host big-integer arithmetic.
+Big product_mod_q(const std::vector<uint64_t>& primes, size_t skip, const Big& qb) {
+    Big Mj = Big::from_u64(1, 80);
+    for (size_t o = 0; o < primes.size(); ++o)
+        if (o != skip) Mj.mul_u64(primes[o]);
+    return mod(Mj, qb);
+}
+
+// prod (device-made, may be null): [k + 1][kMaxLimbs], (M / p_j) mod q then
+// M mod q; the first and last products are recomputed here and must agree.
 void crt_common(const std::vector<uint64_t>& primes, const Big& M, const Big& qb, uint32_t L, uint32_t q_bits,
-                std::vector<uint32_t>& miq, std::vector<uint64_t>& y, QConst& qc) {
+                std::vector<uint32_t>& miq, std::vector<uint64_t>& y, QConst& qc,
+                const std::vector<uint32_t>* prod = nullptr) {
     const size_t k = primes.size();
     miq.assign(k * kMaxLimbs, 0);
     y.assign(k, 0);
+    if (prod && prod->size() != (k + 1) * kMaxLimbs) throw std::runtime_error("self test: device CRT products shape");
     for (size_t j = 0; j < k; ++j) {
         const uint64_t p = primes[j];
-        Big Mj = Big::from_u64(1, 80);
         uint64_t mj_mod_p = 1;
-        for (size_t o = 0; o < k; ++o) {
-            if (o == j) continue;
-            Mj.mul_u64(primes[o]);
-            mj_mod_p = mulmod(mj_mod_p, primes[o] % p, p);
-        }
+        for (size_t o = 0; o < k; ++o)
+            if (o != j) mj_mod_p = mulmod(mj_mod_p, primes[o] % p, p);
         y[j] = invmod(mj_mod_p, p);
-        Big r = mod(Mj, qb);
-        for (uint32_t l = 0; l < L; ++l) miq[j * kMaxLimbs + l] = r.v[l];
+        if (prod) {
+            for (uint32_t l = 0; l < L; ++l) miq[j * kMaxLimbs + l] = (*prod)[j * kMaxLimbs + l];
+        } else {
+            Big r = product_mod_q(primes, j, qb);
+            for (uint32_t l = 0; l < L; ++l) miq[j * kMaxLimbs + l] = r.v[l];
+        }
+    }
+    if (prod) {
+        for (size_t j : {(size_t)0, k - 1}) {
+            Big r = product_mod_q(primes, j, qb);
+            for (uint32_t l = 0; l < L; ++l)
+                if (miq[j * kMaxLimbs + l] != r.v[l]) throw std::runtime_error("self test: device CRT products");
+        }
     }
     std::memset(&qc, 0, sizeof qc);
     for (uint32_t l = 0; l < L; ++l) qc.q[l] = qb.v[l];
     Big mq = mod(M, qb);
+    if (prod)
+        for (uint32_t l = 0; l < L; ++l)
+            if ((*prod)[k * kMaxLimbs + l] != (l < mq.v.size() ? mq.v[l] : 0u))
+                throw std::runtime_error("self test: device M mod q");
     Big neg(L);
     if (mq.bits() != 0) {
         neg = qb;
@@ -146,10 +169,30 @@ void crt_common(const std::vector<uint64_t>& primes, const Big& M, const Big& qb
     qc.L = L;
 }
 
-void build62(HostTables& t, const Big& qb) {
+// Device-chosen primes: every one checked here, and the product must meet
+// the same coefficient bound the host rule stops at.
+void adopt_primes(const std::vector<uint64_t>& primes, uint32_t N, uint32_t log_n, uint32_t q_bits, int bits,
+                  size_t max_k, Big& M) {
+    if (primes.empty() || primes.size() > max_k) throw std::runtime_error("self test: device prime count");
+    M = Big::from_u64(1, 80);
+    for (size_t j = 0; j < primes.size(); ++j) {
+        const uint64_t p = primes[j];
+        if (!is_prime(p) || (p - 1) % (2ull * N) != 0 || p >= (1ull << bits) || (j && p >= primes[j - 1]))
+            throw std::runtime_error("self test: device prime search");
+        M.mul_u64(p);
+    }
+    if ((uint32_t)M.bits() < log_n + 2 * q_bits + 4) throw std::runtime_error("self test: device primes too few");
+}
+
+void build62(HostTables& t, const Big& qb, const DeviceInit* init) {
     HostSet62& s = t.s62;
     Big M;
-    s.primes = pick_primes(t.N, t.log_n, t.q_bits, 62, (size_t)kMaxPrimes, M);
+    if (init) {
+        s.primes = init->primes62;
+        adopt_primes(s.primes, t.N, t.log_n, t.q_bits, 62, (size_t)kMaxPrimes, M);
+    } else {
+        s.primes = pick_primes(t.N, t.log_n, t.q_bits, 62, (size_t)kMaxPrimes, M);
+    }
     const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
     if (t.host_twiddles) {
         s.tw_fwd.resize((size_t)k * N);
@@ -163,10 +206,10 @@ void build62(HostTables& t, const Big& qb) {
     // constants, so the transform emits the CRT-weighted residue
     // v_j = (c mod p_j) y_j mod p_j that the CRT consumes.
     std::vector<uint64_t> y;
-    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc, init ? &init->prod62 : nullptr);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
-        RootTables r = roots(p, N, t.log_n, 64, t.host_twiddles);
+        RootTables r = roots(p, N, t.log_n, 64, t.host_twiddles, init ? init->psi62[j] : 0);
         r.nr = mulmod(r.nr, y[j], p);
         r.wr = mulmod(r.wr, y[j], p);
         for (uint32_t i = 0; i < t.log_n; ++i) {
@@ -221,7 +264,7 @@ void build62(HostTables& t, const Big& qb) {
 // Y = sum_b 2^(8b) D_b + 128 SB 0x01010101 + C_lambda = sum_b 2^(8b) X_b
 // exactly, with C_lambda = sum_b 2^(8b) (128 SA - 128^2 Kdim): the limb sums
 // then carry into the exact non-negative sum_t 2^(8t) X_t < 2^(32 L + 17).
-void build_crt_mfma(HostTables& t, const Big& qb) {
+void build_crt_mfma(HostTables& t, const Big& qb, const DeviceInit* init) {
     HostSet30& s = t.s30;
     const uint32_t L = t.L, k = (uint32_t)s.primes.size();
     // sum_t 2^(8t) X_t < 2^(32 L + 17) must stay below the Barrett range 2^(bits(q) + 71)
@@ -231,6 +274,17 @@ void build_crt_mfma(HostTables& t, const Big& qb) {
     const uint32_t kbytes = 32 * kc;
     s.crt_rt = rt;
     s.crt_kc = kc;
+    if (init && !init->crt_tab.empty()) {  // made on the device; the self tests emulate the contraction over it
+        if (init->crt_tab.size() != (size_t)rt * 32 * kbytes || init->crt_c.size() != (size_t)8 * rt ||
+            init->neg_m_r.size() != (size_t)kMaxLimbs)
+            throw std::runtime_error("self test: device CRT table shape");
+        s.crt_tab = init->crt_tab;
+        s.crt_c = init->crt_c;
+        std::memset(&s.crt_mc, 0, sizeof s.crt_mc);
+        for (uint32_t l = 0; l < L; ++l) s.crt_mc.neg_m_r[l] = init->neg_m_r[l];
+        s.crt_mc.q_inv = (uint32_t)neg_inv(qb.v[0], 32);
+        return;
+    }
     s.crt_tab.assign((size_t)rt * 32 * kbytes, 0);
     s.crt_c.assign((size_t)8 * rt, 0);
     auto row_of = [](uint32_t lam, uint32_t b) { return 32 * (lam / 8) + 8 * ((lam % 8) / 2) + 4 * (lam % 2) + b; };
@@ -280,10 +334,15 @@ void build_crt_mfma(HostTables& t, const Big& qb) {
     s.crt_mc.q_inv = (uint32_t)neg_inv(qb.v[0], 32);
 }
 
-void build30(HostTables& t, const Big& qb) {
+void build30(HostTables& t, const Big& qb, const DeviceInit* init) {
     HostSet30& s = t.s30;
     Big M;
-    s.primes = pick_primes(t.N, t.log_n, t.q_bits, 30, (size_t)kMaxPrimes30, M);
+    if (init) {
+        s.primes = init->primes30;
+        adopt_primes(s.primes, t.N, t.log_n, t.q_bits, 30, (size_t)kMaxPrimes30, M);
+    } else {
+        s.primes = pick_primes(t.N, t.log_n, t.q_bits, 30, (size_t)kMaxPrimes30, M);
+    }
     const uint32_t k = (uint32_t)s.primes.size(), N = t.N;
     if (t.host_twiddles) {
         s.tw_fwd.resize((size_t)k * N);
@@ -294,10 +353,10 @@ void build30(HostTables& t, const Big& qb) {
     s.place.assign((size_t)k * kMaxLimbs, 0);
     s.cc.resize(k);
     std::vector<uint64_t> y;  // CRT weight, merged as in build62
-    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc);
+    crt_common(s.primes, M, qb, t.L, t.q_bits, s.miq, y, s.qc, init ? &init->prod30 : nullptr);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
-        RootTables r = roots(p, N, t.log_n, 32, t.host_twiddles);
+        RootTables r = roots(p, N, t.log_n, 32, t.host_twiddles, init ? init->psi30[j] : 0);
         r.nr = mulmod(r.nr, y[j], p);
         r.wr = mulmod(r.wr, y[j], p);
         for (uint32_t i = 0; i < t.log_n; ++i) {
@@ -356,7 +415,7 @@ void build30(HostTables& t, const Big& qb) {
             for (int u = 0; u < 4; ++u) s.mfma_bias[row0 + u] = (int32_t)bias[u];
         }
     }
-    build_crt_mfma(t, qb);
+    build_crt_mfma(t, qb, init);
     for (uint32_t j = 0; j < k; ++j) {
         const uint64_t p = s.primes[j];
         CrtConst32& cc = s.cc[j];
@@ -371,11 +430,38 @@ void build30(HostTables& t, const Big& qb) {
 
 }  // namespace
 
-HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles) {
-    // validation (solve.cu:1167-1172; N cap raised to 65536)
-    if (N < 2 || N > (1u << kMaxLogN) || (N & (N - 1))) throw std::invalid_argument("N must be a power of two from 2 to 65536");
+bool select_primes(const std::vector<std::pair<uint64_t, uint64_t>>& cands, uint32_t log_n, uint32_t q_bits,
+                   size_t max_k, std::vector<uint64_t>& primes, std::vector<uint64_t>& psi) {
+    const uint32_t bound_bits = log_n + 2 * q_bits + 4;
+    Big M = Big::from_u64(1, 80);
+    primes.clear();
+    psi.clear();
+    for (const auto& c : cands) {
+        if ((uint32_t)M.bits() >= bound_bits) break;
+        if (!c.second) continue;
+        if (primes.size() >= max_k) return false;
+        primes.push_back(c.first);
+        psi.push_back(c.second);
+        M.mul_u64(c.first);
+    }
+    return (uint32_t)M.bits() >= bound_bits;
+}
+
+void validate_modulus(uint32_t L, const uint32_t* q) {
     if (L == 0 || L > (uint32_t)kMaxLimbs) throw std::invalid_argument("L must be in [1, 32]");
     if (!q || !(q[0] & 1u)) throw std::invalid_argument("q must be an odd modulus of L limbs");
+    Big qb(L);
+    for (uint32_t l = 0; l < L; ++l) qb.v[l] = q[l];
+    const int bits = qb.bits();
+    if (bits < 2) throw std::invalid_argument("q must be an odd modulus > 1");
+    // fast-CRT width (solve.cu:1339-1341): bits(q) + 70 <= 34 * 32
+    if (bits + 70 > 1088) throw std::invalid_argument("q wider than 1018 bits is not supported");
+}
+
+HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twiddles, const DeviceInit* init) {
+    // validation (solve.cu:1167-1172; N cap raised to 65536)
+    if (N < 2 || N > (1u << kMaxLogN) || (N & (N - 1))) throw std::invalid_argument("N must be a power of two from 2 to 65536");
+    validate_modulus(L, q);
     HostTables t;
     t.N = N;
     t.L = L;
@@ -387,8 +473,10 @@ HostTables build_tables(uint32_t N, uint32_t L, const uint32_t* q, bool host_twi
     if (t.q_bits < 2) throw std::invalid_argument("q must be an odd modulus > 1");
     // fast-CRT width (solve.cu:1339-1341): bits(q) + 70 <= 34 * 32
     if (t.q_bits + 70 > 1088) throw std::invalid_argument("q wider than 1018 bits is not supported");
-    build62(t, qb);
-    build30(t, qb);
+    if (init && (init->psi62.size() != init->primes62.size() || init->psi30.size() != init->primes30.size()))
+        throw std::runtime_error("self test: device psi count");
+    build62(t, qb, init);
+    build30(t, qb, init);
     return t;
 }
 
