@@ -272,6 +272,7 @@ def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
     wd, wa, ws = mix.stream(w, rate, seed + 1)
     mix.replay(srv, wd, wa, ws, mix.outputs(wd, block)[0])
     outs, _ = mix.outputs(degrees, block)
+    srv.reset_stats()  # the stats below cover the timed replay only
     lat, makespan = mix.replay(srv, degrees, arrivals, sets, outs)
     st = srv.stats()
     wrong = mix.wrong(degrees, sets, outs)
@@ -328,6 +329,9 @@ def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
         "rate_per_s": rate, "throughput_per_s": n / makespan, "makespan_s": makespan,
         "launches": st["launches"], "max_batch_seen": st["max_batch_seen"], "peak_admitted_bytes": st["peak_bytes"],
         "arena_bytes": st["device_bytes"], "admission_waits": st["admission_waits"],
+        # where the slowest requests' time went (maxima over the timed replay)
+        "max_turn_gap_us": st["max_turn_gap_us"], "max_enqueue_us": st["max_enqueue_us"],
+        "max_batch_us": st["max_batch_us"],
         "responses_checked": n, "wrong_responses": wrong,
         "plain_host_to_host_per_s": plain_rps, "over_plain": (n / makespan) / plain_rps,
         "bit_exact_vs_plain": wrong == 0 and all(x["wrong_responses"] == 0 for x in sweep)

@@ -104,9 +104,13 @@ struct Arena {
 
 struct Lane {
     hipStream_t s = nullptr;
-    hipEvent_t ev = nullptr;
+    // one event per request, recorded after its own result copy: a request
+    // completes when its result is home, not when its batch's last one is
+    std::vector<hipEvent_t> req_ev;
+    size_t done_upto = 0;  // requests of the batch already completed (they complete in order)
     std::vector<Req*> batch;
     uint64_t off = 0, bytes = 0;  // the batch's arena block
+    double t_enqueue = 0;         // when the batch's work was enqueued
     bool busy = false;
 };
 
@@ -161,22 +165,52 @@ namespace {
 
 int hip_err(hipError_t e, const char* what) { return set_error(NC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)); }
 
-void finish(nc_server* sv, Lane& ln, int status) {
+// Completes the batch's requests [done_upto, upto) with `status`; once all
+// are complete the lane's arena block returns and the lane goes idle.
+void complete(nc_server* sv, Lane& ln, size_t upto, int status) {
+    if (upto <= ln.done_upto) return;
     const double t = now_s();
     {
         std::lock_guard<std::mutex> g(sv->mu);
-        for (Req* r : ln.batch) {
+        for (size_t i = ln.done_upto; i < upto; ++i) {
+            Req* r = ln.batch[i];
             r->t_done = t;
             r->status = status;
             r->done = true;
         }
-        sv->st.completed += ln.batch.size();
+        sv->st.completed += upto - ln.done_upto;
+        if (upto == ln.batch.size())
+            sv->st.max_batch_us = std::max<uint64_t>(sv->st.max_batch_us, (uint64_t)((t - ln.t_enqueue) * 1e6));
     }
     sv->cv_done.notify_all();
+    ln.done_upto = upto;
+    if (upto < ln.batch.size()) return;
     sv->arena.release(ln.off, ln.bytes);
     ln.bytes = 0;
     ln.batch.clear();
+    ln.done_upto = 0;
     ln.busy = false;
+}
+
+void finish(nc_server* sv, Lane& ln, int status) { complete(sv, ln, ln.batch.size(), status); }
+
+// Completes the lane's requests whose result copies have landed; returns
+// true if any did (or an error ended the batch).
+bool poll_lane(nc_server* sv, Lane& ln) {
+    size_t upto = ln.done_upto;
+    while (upto < ln.batch.size()) {
+        const hipError_t q = hipEventQuery(ln.req_ev[upto]);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) {
+            complete(sv, ln, upto, NC_OK);
+            finish(sv, ln, hip_err(q, "lane"));
+            return true;
+        }
+        ++upto;
+    }
+    if (upto == ln.done_upto) return false;
+    complete(sv, ln, upto, NC_OK);
+    return true;
 }
 
 // Enqueues one batch on an idle lane: the largest batch (up to max_batch)
@@ -193,6 +227,7 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
     }
     const size_t words = d.words;
     const Degree::Parts pt = d.parts(sv->arena.base + ln.off, n);
+    ln.t_enqueue = now_s();
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
         e = hipMemcpyAsync(pt.a + i * words, ln.batch[i]->a, words * 4, hipMemcpyHostToDevice, ln.s);
@@ -203,14 +238,12 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
         rc = nc::run_pipeline_ws(d.plan, pt.ws_a, pt.ws_b, pt.a, pt.b, pt.c, n, ln.s, nc::choose_for(d.plan, n));
     for (uint32_t i = 0; i < n && rc == NC_OK; ++i) {
         e = hipMemcpyAsync(ln.batch[i]->c, pt.c + i * words, words * 4, hipMemcpyDeviceToHost, ln.s);
+        if (e == hipSuccess) e = hipEventRecord(ln.req_ev[i], ln.s);
         if (e != hipSuccess) rc = hip_err(e, "request copy out");
-    }
-    if (rc == NC_OK) {
-        e = hipEventRecord(ln.ev, ln.s);
-        if (e != hipSuccess) rc = hip_err(e, "lane event");
     }
     {
         std::lock_guard<std::mutex> g(sv->mu);
+        sv->st.max_enqueue_us = std::max<uint64_t>(sv->st.max_enqueue_us, (uint64_t)((now_s() - ln.t_enqueue) * 1e6));
         sv->st.launches += 1;
         sv->st.max_batch_seen = std::max<uint64_t>(sv->st.max_batch_seen, n);
         sv->st.peak_bytes = std::max<uint64_t>(sv->st.peak_bytes, sv->arena.used);
@@ -294,10 +327,18 @@ void scheduler(nc_server* sv) {
     }
     sv->cv_done.notify_all();
     if (sv->warm_rc != NC_OK) return;
+    // a gap between turns counts only when work was pending at the end of the
+    // earlier turn (an idle scheduler sleeps up to 2 ms on purpose)
+    double last_turn = now_s();
+    bool was_active = false;
     for (;;) {
         bool any_busy = false, waiting = false;
+        const double turn = now_s();
         {
             std::lock_guard<std::mutex> g(sv->mu);
+            if (was_active)
+                sv->st.max_turn_gap_us =
+                    std::max<uint64_t>(sv->st.max_turn_gap_us, (uint64_t)((turn - last_turn) * 1e6));
             while (!sv->inbox.empty()) {
                 Req* r = sv->inbox.front();
                 sv->inbox.pop_front();
@@ -306,17 +347,8 @@ void scheduler(nc_server* sv) {
         }
         bool progress = false;
         for (auto& kv : sv->deg)
-            for (Lane& ln : kv.second.lanes) {
-                if (!ln.busy) continue;
-                const hipError_t q = hipEventQuery(ln.ev);
-                if (q == hipSuccess) {
-                    finish(sv, ln, NC_OK);
-                    progress = true;
-                } else if (q != hipErrorNotReady) {
-                    finish(sv, ln, hip_err(q, "lane"));
-                    progress = true;
-                }
-            }
+            for (Lane& ln : kv.second.lanes)
+                if (ln.busy && poll_lane(sv, ln)) progress = true;
         for (auto& kv : sv->deg) {
             Degree& d = kv.second;
             for (Lane& ln : d.lanes) {
@@ -333,6 +365,8 @@ void scheduler(nc_server* sv) {
             for (Lane& ln : d.lanes) any_busy = any_busy || ln.busy;
             waiting = waiting || !d.queue.empty();
         }
+        last_turn = now_s();
+        was_active = any_busy || waiting;
         std::unique_lock<std::mutex> lk(sv->mu);
         if (sv->stop && !any_busy && !waiting && sv->inbox.empty()) break;
         if (!progress && sv->inbox.empty()) {
@@ -345,7 +379,8 @@ void scheduler(nc_server* sv) {
 void free_degree(Degree& d) {
     for (Lane& ln : d.lanes) {
         if (ln.s) (void)hipStreamSynchronize(ln.s);
-        if (ln.ev) (void)hipEventDestroy(ln.ev);
+        for (hipEvent_t ev : ln.req_ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (ln.s) (void)hipStreamDestroy(ln.s);
     }
     if (d.plan) nc_plan_destroy(d.plan);
@@ -423,7 +458,9 @@ int nc_server_create(const nc_server_config* cfg, nc_server** out) {
         d.lanes.resize(cfg->lanes_per_degree);
         for (Lane& ln : d.lanes) {
             e = hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.ev, hipEventDisableTiming);
+            ln.req_ev.assign(d.cap, nullptr);
+            for (uint32_t i = 0; i < d.cap && e == hipSuccess; ++i)
+                e = hipEventCreateWithFlags(&ln.req_ev[i], hipEventDisableTiming);
             if (e != hipSuccess) return fail_out(hip_err(e, "lane streams"));
         }
     }
@@ -502,6 +539,15 @@ int nc_server_get_stats(const nc_server* sv, nc_server_stats* out) {
     if (!sv || !out) return set_error(NC_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(const_cast<nc_server*>(sv)->mu);
     *out = sv->st;
+    return NC_OK;
+}
+
+int nc_server_reset_stats(nc_server* sv) {
+    if (!sv) return set_error(NC_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    const uint64_t dev = sv->st.device_bytes;
+    sv->st = nc_server_stats{};
+    sv->st.device_bytes = dev;
     return NC_OK;
 }
 

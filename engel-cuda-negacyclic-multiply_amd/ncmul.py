@@ -66,7 +66,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "launches", "max_batch_seen", "peak_bytes",
-                                                "admission_waits", "device_bytes")]
+                                                "admission_waits", "device_bytes", "max_turn_gap_us",
+                                                "max_enqueue_us", "max_batch_us")]
 
 
 _lib = None
@@ -135,6 +136,7 @@ def lib():
         L.nc_server_submit.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_uint64)]
         L.nc_server_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.nc_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.nc_server_reset_stats.argtypes = [_vp]
         L.nc_server_replay.argtypes = [_vp, ctypes.c_uint32, _u32p, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -541,6 +543,9 @@ class Server:
         st = ServerStats()
         _check(lib().nc_server_get_stats(self._h, ctypes.byref(st)), "nc_server_get_stats")
         return {n: int(getattr(st, n)) for n, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().nc_server_reset_stats(self._h), "nc_server_reset_stats")
 
     def replay(self, degrees, arrivals, a_list, b_list, c_list):
         """Open-loop replay; returns (latencies seconds, makespan seconds)."""

@@ -59,7 +59,8 @@ void nc_graph_destroy(nc_graph* graph);
  * keeps up to `lanes_per_degree` batches of each degree in flight on their
  * own streams (continuous batching: requests that arrive while others run
  * join the next batch as lanes free up); each batch's copies in and out
- * overlap other batches' kernels.  The device memory of requests is one
+ * overlap other batches' kernels.  A request completes as soon as its own
+ * result copy has landed, not when its batch's last one has.  The device memory of requests is one
  * arena the server allocates at creation: hbm_budget bytes, or less when
  * every lane of every degree holding max_batch requests at once needs less
  * (nc_server_stats.device_bytes reports it).  Each batch's operands, results
@@ -89,6 +90,14 @@ typedef struct nc_server_stats {
     uint64_t peak_bytes;         /* most arena bytes in use at once          */
     uint64_t admission_waits;    /* times a waiting request did not fit      */
     uint64_t device_bytes;       /* the request arena's size                 */
+    /* where latency goes (microseconds, maxima since creation or the last
+     * nc_server_reset_stats): the longest gap between two scheduler turns,
+     * the longest host time inside one batch's enqueue (copies, kernels,
+     * events), and the longest time from a batch's enqueue to the turn that
+     * saw its last request complete */
+    uint64_t max_turn_gap_us;
+    uint64_t max_enqueue_us;
+    uint64_t max_batch_us;
 } nc_server_stats;
 
 typedef struct nc_server nc_server;
@@ -113,6 +122,9 @@ int nc_server_submit(nc_server* server, uint32_t N, const uint32_t* a, const uin
  * A ticket can be waited for once. */
 int nc_server_wait(nc_server* server, uint64_t ticket, int timeout_ms, double* latency_s);
 int nc_server_get_stats(const nc_server* server, nc_server_stats* out);
+/* Zeroes every counter and maximum of nc_server_stats except device_bytes
+ * (to measure one stretch of traffic, e.g. after a warm-up). */
+int nc_server_reset_stats(nc_server* server);
 /* Open-loop replay: request i is submitted at arrival_s[i] seconds after the
  * call starts (arrival_s ascending); latency_s[i] receives its completion
  * time minus its scheduled arrival, makespan_s (nullable) the time from start
