@@ -331,7 +331,7 @@ def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
         "arena_bytes": st["device_bytes"], "admission_waits": st["admission_waits"],
         # where the slowest requests' time went (maxima over the timed replay)
         "max_turn_gap_us": st["max_turn_gap_us"], "max_enqueue_us": st["max_enqueue_us"],
-        "max_batch_us": st["max_batch_us"],
+        "max_batch_us": st["max_batch_us"], "mapped_requests": st["mapped_requests"],
         "responses_checked": n, "wrong_responses": wrong,
         "plain_host_to_host_per_s": plain_rps, "over_plain": (n / makespan) / plain_rps,
         "bit_exact_vs_plain": wrong == 0 and all(x["wrong_responses"] == 0 for x in sweep)

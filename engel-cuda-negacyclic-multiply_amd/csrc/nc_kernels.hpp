@@ -120,6 +120,15 @@ hipError_t launch_crt_mfma_tables(const uint32_t* miq, const uint64_t* primes, u
                                   uint32_t L, uint32_t rt, uint32_t kc, const uint32_t* neg_m, int8_t* tab,
                                   int64_t* corr, uint32_t* neg_m_r, hipStream_t s);
 
+// Moves count buffers of `bytes` bytes each (a multiple of 8; every pointer
+// 8-byte aligned) from src[i] to dst[i] with the GPU's own loads and stores:
+// to_host, device memory -> mapped pinned host memory; else the reverse.
+// Host pointers must be device-visible addresses of mapped pinned memory
+// whose allocation holds the whole buffer.
+constexpr uint32_t kHostXferMax = 64;  // buffers per launch
+hipError_t launch_host_xfer(const void* const* src, void* const* dst, uint32_t count, uint64_t bytes, bool to_host,
+                            hipStream_t s);
+
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 

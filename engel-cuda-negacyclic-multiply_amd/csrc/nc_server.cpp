@@ -11,10 +11,13 @@
 // scheduler (1) moves newly submitted requests to their degree's queue,
 // (2) retires lanes whose event has fired (requests complete, their block
 // returns to the arena), and (3) fills every idle lane with waiting requests
-// of its degree, as many as max_batch and the arena allow: H2D copies of the
-// operands, the batched pipeline, D2H copies of the results, then the lane's
-// event, all on the lane's stream, so copies of one batch overlap kernels of
-// others.  Requests that arrive while lanes are busy are admitted as soon as
+// of its degree, as many as max_batch and the arena allow: the operands in,
+// the batched pipeline, each result out followed by its own event, all on
+// the lane's stream, so transfers of one batch overlap kernels of others.
+// Requests below kMapMaxBytes per buffer whose buffers are mapped pinned
+// memory move by the GPU's own loads and stores (one gather launch for all
+// operands of a batch, one scatter launch per result); the others by
+// hipMemcpyAsync.  Requests that arrive while lanes are busy are admitted as soon as
 // a lane and arena space free up (the next turn), not after the queue
 // drains.
 #include <hip/hip_runtime.h>
@@ -44,12 +47,47 @@ double now_s() { return std::chrono::duration<double>(Clock::now().time_since_ep
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~255ull; }
 
+// Buffers below this size move by the transfer kernels, larger ones by
+// hipMemcpyAsync.  DMA copies pay a fixed ~13 us each (240 KB copies reach
+// ~13 GB/s) but take no compute units and reach 45-53 GB/s from 2 MB up,
+// where transfer kernels would compete with the pipeline's kernels for
+// compute-unit slots.  On MI355X, serving degrees 2^11..2^16 at 6000
+// requests/s: this split gave p50 0.37 ms and p99 2.3-3.1 ms against 0.53-0.64
+// and 7.5-9.8 ms with copies only and 0.64-0.67 and 3.8-4.7 ms with kernels
+// only, and held p99 near 4 ms at 8000/s, where either alone reached 18-41 ms.
+constexpr uint64_t kMapMaxBytes = 1u << 20;
+
+// Device-visible address of [p, p + bytes) if it lies in one mapped pinned
+// host allocation and is 8-byte aligned (what the transfer kernels need),
+// else null.  HIP maps pinned memory at its host address; anything else
+// (pageable memory, device memory, another mapping) takes the copy path.
+// One allocation: the first and the last word are both mapped pinned memory
+// with the same range start (the range size attribute holds only 32 bits:
+// it reads 0 for a 24 GiB allocation).
+const void* mapped(const void* p, uint64_t bytes) {
+    if (reinterpret_cast<uintptr_t>(p) % 8 || bytes % 8 || bytes == 0) return nullptr;
+    void* start[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k) {
+        const char* x = static_cast<const char*>(p) + (k ? bytes - 8 : 0);
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, x) != hipSuccess || at.type != hipMemoryTypeHost || at.devicePointer != x ||
+            hipPointerGetAttribute(&start[k], HIP_POINTER_ATTRIBUTE_RANGE_START_ADDR, const_cast<char*>(x)) != hipSuccess)
+            return nullptr;
+    }
+    return start[0] && start[0] == start[1] ? p : nullptr;
+}
+
 struct Req {
     uint64_t id = 0;
     uint32_t N = 0;
     const uint32_t* a = nullptr;
     const uint32_t* b = nullptr;
     uint32_t* c = nullptr;
+    // device-visible addresses of a, b, c when all three are mapped pinned
+    // memory (the transfer kernels move them), else null (copies do)
+    const void* da = nullptr;
+    const void* db = nullptr;
+    void* dc = nullptr;
     uint64_t bytes = 0;
     double t_submit = 0, t_done = 0;
     bool done = false;
@@ -109,6 +147,8 @@ struct Lane {
     std::vector<hipEvent_t> req_ev;
     size_t done_upto = 0;  // requests of the batch already completed (they complete in order)
     std::vector<Req*> batch;
+    std::vector<const void*> xsrc;  // gather lists, kept to spare allocations
+    std::vector<void*> xdst;
     uint64_t off = 0, bytes = 0;  // the batch's arena block
     double t_enqueue = 0;         // when the batch's work was enqueued
     bool busy = false;
@@ -229,23 +269,39 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
     const Degree::Parts pt = d.parts(sv->arena.base + ln.off, n);
     ln.t_enqueue = now_s();
     hipError_t e = hipSuccess;
+    ln.xsrc.clear();
+    ln.xdst.clear();
+    uint64_t n_mapped = 0;
     for (uint32_t i = 0; i < n && e == hipSuccess; ++i) {
-        e = hipMemcpyAsync(pt.a + i * words, ln.batch[i]->a, words * 4, hipMemcpyHostToDevice, ln.s);
-        if (e == hipSuccess) e = hipMemcpyAsync(pt.b + i * words, ln.batch[i]->b, words * 4, hipMemcpyHostToDevice, ln.s);
+        const Req* r = ln.batch[i];
+        if (r->dc) {
+            ln.xsrc.insert(ln.xsrc.end(), {r->da, r->db});
+            ln.xdst.insert(ln.xdst.end(), {pt.a + i * words, pt.b + i * words});
+            ++n_mapped;
+            continue;
+        }
+        e = hipMemcpyAsync(pt.a + i * words, r->a, words * 4, hipMemcpyHostToDevice, ln.s);
+        if (e == hipSuccess) e = hipMemcpyAsync(pt.b + i * words, r->b, words * 4, hipMemcpyHostToDevice, ln.s);
     }
-    int rc = e == hipSuccess ? NC_OK : hip_err(e, "request copy in");
+    if (e == hipSuccess && !ln.xsrc.empty())
+        e = nc::launch_host_xfer(ln.xsrc.data(), ln.xdst.data(), (uint32_t)ln.xsrc.size(), words * 4, false, ln.s);
+    int rc = e == hipSuccess ? NC_OK : hip_err(e, "request operands in");
     if (rc == NC_OK)
         rc = nc::run_pipeline_ws(d.plan, pt.ws_a, pt.ws_b, pt.a, pt.b, pt.c, n, ln.s, nc::choose_for(d.plan, n));
     for (uint32_t i = 0; i < n && rc == NC_OK; ++i) {
-        e = hipMemcpyAsync(ln.batch[i]->c, pt.c + i * words, words * 4, hipMemcpyDeviceToHost, ln.s);
+        Req* r = ln.batch[i];
+        const void* src = pt.c + i * words;
+        e = r->dc ? nc::launch_host_xfer(&src, &r->dc, 1, words * 4, true, ln.s)
+                  : hipMemcpyAsync(r->c, src, words * 4, hipMemcpyDeviceToHost, ln.s);
         if (e == hipSuccess) e = hipEventRecord(ln.req_ev[i], ln.s);
-        if (e != hipSuccess) rc = hip_err(e, "request copy out");
+        if (e != hipSuccess) rc = hip_err(e, "request result out");
     }
     {
         std::lock_guard<std::mutex> g(sv->mu);
         sv->st.max_enqueue_us = std::max<uint64_t>(sv->st.max_enqueue_us, (uint64_t)((now_s() - ln.t_enqueue) * 1e6));
         sv->st.launches += 1;
         sv->st.max_batch_seen = std::max<uint64_t>(sv->st.max_batch_seen, n);
+        sv->st.mapped_requests += n_mapped;
         sv->st.peak_bytes = std::max<uint64_t>(sv->st.peak_bytes, sv->arena.used);
     }
     if (rc != NC_OK) {
@@ -258,11 +314,11 @@ bool dispatch(nc_server* sv, Degree& d, Lane& ln) {
 }
 
 // Warms every lane before the first request, on the scheduler thread (the
-// thread that issues all lane work later): copies in from pinned memory,
-// the pipeline at every batch-size bucket (1, 2, 4, .., max_batch, or the
+// thread that issues all lane work later): copies and transfer kernels in
+// from pinned memory, the pipeline at every batch-size bucket (1, 2, 4, .., max_batch, or the
 // largest whose block fits the arena: each bucket's tuning may use kernels
 // nothing has launched yet, and a kernel's first launch in a process is
-// slow), the copy out, all on the lane's own stream, in arena blocks that
+// slow), copy and transfer kernel out, all on the lane's own stream, in arena blocks that
 // are returned afterwards.  Measured at 6000 requests/s: without it the
 // first ~0.6 s of a process ran behind (p99 ~40 ms, against ~4 ms later);
 // with it the first ~0.1 s still does.
@@ -274,6 +330,8 @@ int warm_lanes(nc_server* sv) {
     hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&h), 3 * max_words * 4, hipHostMallocDefault);
     if (e != hipSuccess) return hip_err(e, "warm-up buffer");
     std::fill(h, h + 3 * max_words, 0u);
+    // the same buffer as the transfer kernels see it (null: they are not warmed)
+    uint32_t* hd = static_cast<uint32_t*>(const_cast<void*>(mapped(h, 3 * max_words * 4)));
     int rc = NC_OK;
     for (auto& kv : sv->deg) {
         Degree& d = kv.second;
@@ -292,12 +350,22 @@ int warm_lanes(nc_server* sv) {
                 const Degree::Parts pt = d.parts(block, n);
                 if (e == hipSuccess) e = hipMemcpyAsync(pt.a, h, words * 4, hipMemcpyHostToDevice, ln.s);
                 if (e == hipSuccess) e = hipMemcpyAsync(pt.b, h + max_words, words * 4, hipMemcpyHostToDevice, ln.s);
+                if (e == hipSuccess && hd) {
+                    const void* src[2] = {hd, hd + max_words};
+                    void* dst[2] = {pt.a, pt.b};
+                    e = nc::launch_host_xfer(src, dst, 2, words * 4, false, ln.s);
+                }
                 if (e != hipSuccess) rc = hip_err(e, "warm-up copy in");
                 if (rc == NC_OK)
                     rc = nc::run_pipeline_ws(d.plan, pt.ws_a, pt.ws_b, pt.a, pt.b, pt.c, n, ln.s,
                                              nc::choose_for(d.plan, n));
                 if (rc == NC_OK) {
                     e = hipMemcpyAsync(h + 2 * max_words, pt.c, words * 4, hipMemcpyDeviceToHost, ln.s);
+                    if (e == hipSuccess && hd) {
+                        const void* src = pt.c;
+                        void* dst = hd + 2 * max_words;
+                        e = nc::launch_host_xfer(&src, &dst, 1, words * 4, true, ln.s);
+                    }
                     if (e != hipSuccess) rc = hip_err(e, "warm-up copy out");
                 }
                 if (n == top) break;
@@ -500,6 +568,10 @@ int nc_server_submit(nc_server* sv, uint32_t N, const uint32_t* a, const uint32_
     r->a = a;
     r->b = b;
     r->c = c;
+    const uint64_t io = it->second.words * 4;
+    r->da = io < kMapMaxBytes ? mapped(a, io) : nullptr;
+    r->db = r->da ? mapped(b, io) : nullptr;
+    r->dc = r->db ? const_cast<void*>(mapped(c, io)) : nullptr;
     r->bytes = it->second.footprint;
     r->t_submit = now_s();
     {

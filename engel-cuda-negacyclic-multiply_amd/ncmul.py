@@ -67,7 +67,7 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "launches", "max_batch_seen", "peak_bytes",
                                                 "admission_waits", "device_bytes", "max_turn_gap_us",
-                                                "max_enqueue_us", "max_batch_us")]
+                                                "max_enqueue_us", "max_batch_us", "mapped_requests")]
 
 
 _lib = None

@@ -68,8 +68,13 @@ void nc_graph_destroy(nc_graph* graph);
  * count x nc_server_footprint bytes), and a waiting request is admitted only
  * when its batch's block fits, so the device bytes of admitted, unfinished
  * requests never exceed hbm_budget; plan tables (nc_plan_info.table_bytes
- * per degree) are outside it.  Host buffers from nc_host_alloc (pinned) let copies overlap;
- * other host memory works but copies synchronously. */
+ * per degree) are outside it.  When a request's a, b and c are each below
+ * 1 MiB (N L 4 bytes; N <= 8192 at L = 28) and all lie in mapped pinned
+ * memory (nc_host_alloc, or any hipHostMalloc allocation), the GPU moves
+ * them with its own loads and stores over PCIe: one launch for all operands
+ * of a batch, one per result, with no per-copy DMA set-up.  Larger requests
+ * in pinned memory move by hipMemcpyAsync on the DMA engines, which overlap
+ * kernels; pageable memory works too but copies synchronously. */
 typedef struct nc_server_config {
     uint32_t L;
     const uint32_t* q;
@@ -98,6 +103,7 @@ typedef struct nc_server_stats {
     uint64_t max_turn_gap_us;
     uint64_t max_enqueue_us;
     uint64_t max_batch_us;
+    uint64_t mapped_requests;    /* requests moved by the transfer kernels     */
 } nc_server_stats;
 
 typedef struct nc_server nc_server;

@@ -161,6 +161,40 @@ def test_server_mix_equals_alone(gpu, q868):
     assert np.array_equal(c, O.Plan(N, q868).multiply(np.asarray(a), np.asarray(b)))
 
 
+def test_server_transfer_paths(gpu, q868):
+    """Requests (below 1 MiB per buffer) whose operands and result all lie
+    in mapped pinned memory move by the transfer kernels, the others by
+    copies (pageable memory,
+    pinned memory at an address that is not 8-byte aligned); both kinds in
+    one batch, pinned views at offsets inside a larger allocation included,
+    and every result exact."""
+    N, L = 4096, q868.shape[0]
+    reqs = _requests([N] * 6, q868, 13000)  # all pinned
+    n0, a0, b0, c0 = reqs[1]
+    reqs[1] = (N, a0, b0, np.zeros((N, L), np.uint32))  # pageable result
+    n0, a0, b0, c0 = reqs[2]
+    reqs[2] = (N, np.array(a0), np.array(b0), c0)  # pageable operands
+    odd = ncmul.host_array((N * L + 1,))  # pinned, 4 bytes past 8-byte alignment
+    n0, a0, b0, c0 = reqs[3]
+    odd_c = odd[1:].reshape(N, L)
+    odd_c[:] = 0
+    reqs[3] = (N, a0, b0, odd_c)
+    rows = ncmul.host_array((3, N, L))  # pinned rows at offsets inside one allocation
+    n0, a0, b0, c0 = reqs[4]
+    rows[0], rows[2] = a0, b0
+    reqs[4] = (N, rows[0], rows[2], rows[1])
+    alone = _alone(q868, reqs)
+    with ncmul.Server(q868, [N], max_batch=8, lanes=1, autotune=False) as srv:
+        tickets = [srv.submit(a, b, c) for _, a, b, c in reqs]
+        for t in tickets:
+            srv.wait(t, timeout_ms=60000)
+        st = srv.stats()
+    say("transfer paths", st)
+    assert st["mapped_requests"] == 3  # requests 0, 4 and 5
+    for i, (_, a, b, c) in enumerate(reqs):
+        assert np.array_equal(c, alone[i]), i
+
+
 def test_server_admission_budget(gpu, q868):
     degrees = [4096] * 24 + [8192] * 8
     reqs = _requests(degrees, q868, 7000)
@@ -204,6 +238,8 @@ def test_server_poisson_replay(gpu, q868):
         st = srv.stats()
     say("replay", n, "makespan", makespan, "p50", np.percentile(lat, 50), "p99", np.percentile(lat, 99), st)
     assert st["completed"] == n and (lat > 0).all() and makespan >= arrivals[-1]
+    # every buffer pinned: the requests below 1 MiB per buffer moved by the transfer kernels
+    assert st["mapped_requests"] == sum(N * 28 * 4 < (1 << 20) for N in degrees)
     # every response, each with its own operands and result buffer
     alone = _alone(q868, reqs)
     for i in range(n):
