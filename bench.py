@@ -279,7 +279,7 @@ def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
     # rate sweep on the same server: highest offered rate whose p99 stays
     # within the bound (each point its own seeded stream, every response checked)
     sweep, best = [], None
-    sweep_n = min(n, 2000)
+    sweep_n = min(n, 4000)
     for r in (5000.0, 6000.0, 7000.0, 8000.0, 9000.0, 10000.0):
         dg, ar, se = mix.stream(sweep_n, r, seed + int(r))
         so, _ = mix.outputs(dg, block)
@@ -323,7 +323,8 @@ def served(q: np.ndarray, n: int, rate: float, seed: int, p99_bound_ms: float):
     out = {
         "workload": f"{n} requests, N log-uniform in {{2^11..2^16}} (seed {seed}), L=28, 64 operand pairs per degree, "
                     f"Poisson arrivals at {rate:.0f}/s replayed open loop after an untimed {w}-request warm-up "
-                    f"replay; host operands and a distinct result buffer per request (pinned), copies included; "
+                    f"replay; host operands and a distinct result buffer per request (pinned), transfers included "
+                    f"(buffers below 1 MiB by the GPU's own loads and stores, larger by DMA copies); "
                     f"continuous batching, max batch 32, 2 lanes per degree, HBM-footprint admission (16 GiB "
                     f"budget), per-degree tables resident",
         "rate_per_s": rate, "throughput_per_s": n / makespan, "makespan_s": makespan,
