@@ -1152,8 +1152,13 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 // and 3N/4 away -- tile 0 column c holds x0 + (c & 15) + (c >> 4) N/2,
 // tile 1 the same + N/4 -- so the stage of span N/4 pairs a lane's own two
 // tiles and the stage of span N/2 (N^-1 R y_j merged) pairs lanes c and
-// c ^ 16; the residues become canonical before the contraction, and the
-// results leave as four runs of 16 coefficients.
+// c ^ 16, and the results leave as four runs of 16 coefficients.  The
+// residues enter the contraction lazy, in [0, 2p) rather than canonical:
+// the sum x = sum_j v_j (M / p_j) - K M is the same integer for any
+// representatives, and the bounds hold for every v_j < 2^31 -- F < 2^63,
+// K = round(F / 2^56) <= 127, the limb sum below 2^(32 L + 17) and x < q 2^96
+// for the Montgomery finish -- which spares one conditional subtraction per
+// residue (6-7 % of the kernel's time).
 // One prime's constants for the fused inverse stages as one half of a
 // column group sees them (entry 2 r + u, u = 1 for lanes c with c & 16):
 // the span-N/4 twiddle psi^-bitrev(2 + u), the stage-0 factor (N^-1 R y_j
@@ -1254,7 +1259,7 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
         if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
         load_group(g0);
         if constexpr (INV > 0) {
-            // last two inverse stages, then canonical residues (rows >= k are
+            // last two inverse stages, residues left in [0, 2p) (rows >= k are
             // replaced by 0 at use; their constants are row k - 1's).  The
             // memory clobber keeps the per-prime constants from being hoisted
             // out of the group loop (32 primes x 9 words would not fit).
@@ -1278,8 +1283,8 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
                     const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
                     const uint32_t sa = pa + (a ^ m) + ci.sadd;
                     const uint32_t sb = pb + (b ^ m) + ci.sadd;
-                    bf[0][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p), p);
-                    bf[1][kc][e] = (int)W30::sub_if_ge(Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p), p);
+                    bf[0][kc][e] = (int)Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p);
+                    bf[1][kc][e] = (int)Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p);
                 }
             }
         }
