@@ -587,9 +587,13 @@ void check_decompose_mfma(const HostTables& t, std::mt19937_64& rng) {
 }
 
 // Host emulation of the int8 MFMA CRT contraction (crt30_mfma_kernel) on
-// extreme and random CRT-weighted residues: the q limbs must agree with
-// sum_j v_j ((M / p_j) mod q) mod q, empty limbs must be exactly zero and
-// the fraction limbs must equal sum_j v_j floor(2^56 / p_j).
+// extreme and random CRT-weighted residues, canonical ([0, p_j)) and, for
+// the shape the fused inverse stages run at (L = 28, rt = 4, kc = 8, so
+// k <= 64), lazy ([0, 2 p_j), what those stages hand over): the q limbs must
+// agree with sum_j v_j ((M / p_j) mod q) mod q, empty limbs must be exactly
+// zero and the fraction limbs must equal sum_j v_j floor(2^56 / p_j), below
+// 2^63 (so the kernel's 64-bit fraction and its K = round(F / 2^56) < 128
+// hold).
 void check_crt_mfma(const HostTables& t, std::mt19937_64& rng) {
     const HostSet30& s = t.s30;
     const uint32_t L = t.L, k = (uint32_t)s.primes.size();
@@ -598,11 +602,13 @@ void check_crt_mfma(const HostTables& t, std::mt19937_64& rng) {
     Big qb(L);
     for (uint32_t l = 0; l < L; ++l) qb.v[l] = s.qc.q[l];
     auto row_of = [](uint32_t lam, uint32_t b) { return 32 * (lam / 8) + 8 * ((lam % 8) / 2) + 4 * (lam % 2) + b; };
-    for (int trial = 0; trial < 4; ++trial) {
+    const bool fused_shape = L == 28 && rt == 4 && s.crt_kc == 8;
+    for (int trial = 0; trial < (fused_shape ? 6 : 4); ++trial) {
         std::vector<uint32_t> v(8 * s.crt_kc, 0u);
         for (uint32_t j = 0; j < k; ++j) {
             const uint32_t p = (uint32_t)s.primes[j];
-            v[j] = trial == 0 ? p - 1 : trial == 1 ? 0u : (uint32_t)(rng() % p);
+            v[j] = trial == 0 ? p - 1 : trial == 1 ? 0u : trial == 4 ? 2 * p - 1 : trial == 5 ? (uint32_t)(rng() % (2 * p))
+                                                                                    : (uint32_t)(rng() % p);
         }
         std::vector<int> bq(kbytes);
         int64_t sb = 0;
@@ -638,7 +644,7 @@ void check_crt_mfma(const HostTables& t, std::mt19937_64& rng) {
         const int64_t f0 = Y(lf), f1 = Y(lf + 1);
         u128 fwant = 0;
         for (uint32_t j = 0; j < k; ++j) fwant += (u128)v[j] * (uint64_t)((((u128)1) << 56) / s.primes[j]);
-        if (f0 < 0 || f1 < 0 || (u128)(uint64_t)f0 + ((u128)(uint64_t)f1 << 32) != fwant)
+        if (f0 < 0 || f1 < 0 || (u128)(uint64_t)f0 + ((u128)(uint64_t)f1 << 32) != fwant || (fwant >> 63) != 0)
             throw std::runtime_error("self test: CRT table fraction");
         Big want(L + 3);
         for (uint32_t j = 0; j < k; ++j) {
