@@ -389,7 +389,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=30,
                     help="untimed steps first (the first ~20 batched steps after the autotuner run 2-3 %% slower)")
-    ap.add_argument("--batch", type=int, default=128, help="products per step")
+    ap.add_argument("--batch", type=int, default=256, help="products per step")
     ap.add_argument("--N", type=int, default=N_DEFAULT)
     ap.add_argument("--plain-calls", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
