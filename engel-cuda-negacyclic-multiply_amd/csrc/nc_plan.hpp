@@ -133,6 +133,7 @@ struct nc_plan {
     // staging for the host-pointer path (one product)
     uint32_t* d_io = nullptr;   // a | b | c, N*L u32 each
     uint32_t* h_pin = nullptr;  // pinned a | b | c
+    uint32_t* h_pin_dev = nullptr;  // h_pin as the GPU addresses it (null: copies move it)
     hipStream_t stream = nullptr;
     // chunked batches: two internal streams (default priority) and the events
     // that fork them from, stagger them and join them back to the caller's stream
