@@ -101,6 +101,7 @@ Choice choose(const nc_plan* p, uint32_t batch, bool unchunked = false) {
     if (p->mfma_mode >= 0) c.mfma = (uint32_t)p->mfma_mode;
     if (p->chunks_forced) c.chunks = p->chunks_forced;
     if (unchunked) c.chunks = 1;
+    if (p->persist_mode >= 0) c.persist = (uint32_t)p->persist_mode;
     return c;
 }
 
@@ -171,7 +172,8 @@ int run_chunked(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* 
 
 int run_pipeline(nc_plan* p, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c, uint32_t batch, hipStream_t s,
                  const Choice& c) {
-    if (c.chunks > 1 && batch > 1) return run_chunked(p, d_a, d_b, d_c, batch, s, c);
+    const bool persist = c.persist && nc::persist_available(p->dev, batch);
+    if (c.chunks > 1 && batch > 1 && !persist) return run_chunked(p, d_a, d_b, d_c, batch, s, c);
     return nc::run_pipeline_ws(p, p->d_res_a, p->d_res_b, d_a, d_b, d_c, batch, s, c);
 }
 
@@ -195,11 +197,35 @@ int ws_release(nc_plan* p, hipStream_t s) {
     return NC_OK;
 }
 
+// After a persistent launch of `batch` products on the plan's workspace
+// (stream s): did every XCD run all of its items, with no readiness wait
+// reaching its bound?  Reads the control block and the plan's error word
+// (clearing it).
+bool persist_drained(nc_plan* p, uint32_t batch, hipStream_t s) {
+    if (hipStreamSynchronize(s) != hipSuccess) return false;
+    const uint32_t nxcd = p->dev.xcds, words = nc::persist_xcd_words(batch, nxcd);
+    const size_t rows = (size_t)batch * p->dev.w30.k * p->dev.N * sizeof(uint32_t);
+    std::vector<uint32_t> h((size_t)nxcd * words);
+    uint32_t err = 0;
+    if (hipMemcpy(h.data(), static_cast<char*>(p->d_res_b) + rows, h.size() * sizeof(uint32_t),
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&err, p->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    if (err) (void)hipMemset(p->d_err, 0, sizeof err);
+    const nc::PersistShape sh = nc::persist_shape_for(p->dev.log_n, p->dev.w30.k);
+    bool ok = err == 0;
+    for (uint32_t x = 0; x < nxcd; ++x) {
+        const uint32_t* xb = h.data() + (size_t)x * words;
+        ok = ok && xb[nc::kCtlErr] == 0 && xb[nc::kCtlDone] == nc::persist_products_on(batch, nxcd, x) * sh.R;
+    }
+    return ok;
+}
+
 // ---- persisted autotuner choices ------------------------------------------
 // One text line per tuned (device, shape, batch bucket, pinned settings):
-//   nc-tune-v1 <arch> <cus> <N> <L> <q_bits> <k62> <k30> <bucket> <mfma_mode>
-//   <chunks_forced> : <width> <log_blocks> <mfma> <chunks> <width1>
-//   <log_blocks1> <mfma1> <median_ms>
+//   nc-tune-v2 <arch> <cus> <N> <L> <q_bits> <k62> <k30> <bucket> <mfma_mode>
+//   <chunks_forced> <persist_mode> : <width> <log_blocks> <mfma> <chunks>
+//   <persist> <width1> <log_blocks1> <mfma1> <persist1> <median_ms>
 // (the second group is the unchunked choice).  The file is appended to; the
 // last line for a key wins.  Path: nc_set_tune_cache, else the environment
 // variable NC_TUNE_CACHE, else no cache.
@@ -222,15 +248,15 @@ std::string cache_key(const nc_plan* p, uint32_t batch) {
     for (char& ch : arch)
         if (ch == ' ' || ch == ':') ch = '_';
     char buf[256];
-    std::snprintf(buf, sizeof buf, "nc-tune-v1 %s %d %u %u %u %u %u %u %d %u", arch.c_str(),
+    std::snprintf(buf, sizeof buf, "nc-tune-v2 %s %d %u %u %u %u %u %u %d %u %d", arch.c_str(),
                   prop.multiProcessorCount, p->host.N, p->host.L, p->host.q_bits, k_of(p, 62), k_of(p, 30),
-                  bucket_of(batch), p->mfma_mode, p->chunks_forced);
+                  bucket_of(batch), p->mfma_mode, p->chunks_forced, p->persist_mode);
     return buf;
 }
 
 bool valid_choice(const nc_plan* p, const Choice& c) {
     if (c.width != 30 && c.width != 62) return false;
-    if (c.chunks < 1 || c.chunks > 64 || c.mfma > kMfmaAll) return false;
+    if (c.chunks < 1 || c.chunks > 64 || c.mfma > kMfmaAll || c.persist > 1) return false;
     if (p->host.log_n >= 4 &&
         (c.log_blocks < nc::min_log_blocks(p->host.log_n) || c.log_blocks > nc::max_log_blocks(p->host.log_n)))
         return false;
@@ -251,9 +277,9 @@ bool cache_lookup(const nc_plan* p, const std::string& key, nc_plan::Tuned* out)
         if (!colon || (size_t)(colon - line) != key.size() || std::strncmp(line, key.c_str(), key.size()) != 0) continue;
         nc_plan::Tuned t;
         float med = 0.f;
-        if (std::sscanf(colon + 3, "%u %u %u %u %u %u %u %f", &t.choice.width, &t.choice.log_blocks, &t.choice.mfma,
-                        &t.choice.chunks, &t.unchunked.width, &t.unchunked.log_blocks, &t.unchunked.mfma,
-                        &med) != 8)
+        if (std::sscanf(colon + 3, "%u %u %u %u %u %u %u %u %u %f", &t.choice.width, &t.choice.log_blocks,
+                        &t.choice.mfma, &t.choice.chunks, &t.choice.persist, &t.unchunked.width,
+                        &t.unchunked.log_blocks, &t.unchunked.mfma, &t.unchunked.persist, &med) != 10)
             continue;
         t.unchunked.chunks = 1;
         t.median_ms = med;
@@ -270,9 +296,10 @@ void cache_store(const std::string& key, const nc_plan::Tuned& t) {
     if (path.empty()) return;
     std::lock_guard<std::mutex> g(g_file_mu);
     char buf[512];
-    const int len = std::snprintf(buf, sizeof buf, "%s : %u %u %u %u %u %u %u %.5f\n", key.c_str(), t.choice.width,
-                                  t.choice.log_blocks, t.choice.mfma, t.choice.chunks, t.unchunked.width,
-                                  t.unchunked.log_blocks, t.unchunked.mfma, t.median_ms);
+    const int len = std::snprintf(buf, sizeof buf, "%s : %u %u %u %u %u %u %u %u %u %.5f\n", key.c_str(),
+                                  t.choice.width, t.choice.log_blocks, t.choice.mfma, t.choice.chunks,
+                                  t.choice.persist, t.unchunked.width, t.unchunked.log_blocks, t.unchunked.mfma,
+                                  t.unchunked.persist, t.median_ms);
     // one write of the whole line in append mode: lines from concurrent
     // processes do not interleave
     FILE* f = std::fopen(path.c_str(), "a");
@@ -299,6 +326,18 @@ size_t workspace_bytes_per_product(const nc_plan* p) { return p->res_bytes; }
 
 int run_pipeline_ws(nc_plan* p, void* ws_a, void* ws_b, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
                     uint32_t batch, hipStream_t s, const Choice& c) {
+    if (c.persist && persist_available(p->dev, batch)) {
+        // one launch for the batch; its control block sits after b's
+        // residue rows, in the trailer every product's workspace carries
+        const size_t rows = (size_t)batch * p->dev.w30.k * p->dev.N * sizeof(uint32_t);
+        if (rows + persist_ctl_bytes(p->dev, batch) > (size_t)batch * p->res_bytes)
+            return fail(NC_ERR_CAPACITY, "workspace trailer too small for the persistent control block");
+        uint32_t* ctl = reinterpret_cast<uint32_t*>(static_cast<char*>(ws_b) + rows);
+        NC_HIP(launch_persist30(p->dev, d_a, d_b, d_c, static_cast<uint32_t*>(ws_a), static_cast<uint32_t*>(ws_b),
+                                batch, ctl, p->d_err, p->persist_tmo, s),
+               "persistent launch");
+        return NC_OK;
+    }
     const bool dm = (c.mfma & kMfmaDecompose) != 0;
     const bool fuse = decompose_fuses_outer(p->dev, (int)c.width, dm, c.log_blocks);
     NC_HIP(launch_decompose(p->dev, (int)c.width, d_a, ws_a, batch, d_b, ws_b, batch, s, dm, fuse ? c.log_blocks : 0u),
@@ -367,6 +406,7 @@ void nc_plan_destroy(nc_plan* p) {
     if (p->d_res_a) (void)hipFree(p->d_res_a);
     if (p->d_res_b) (void)hipFree(p->d_res_b);
     if (p->d_io) (void)hipFree(p->d_io);
+    if (p->d_err) (void)hipFree(p->d_err);
     if (p->h_pin) (void)hipHostFree(p->h_pin);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     for (int k = 0; k < 2; ++k) {
@@ -522,6 +562,12 @@ int upload_tables(const nc::HostTables& t, int device, void** d_tables, size_t* 
     int cus = 0;
     NC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     d.cus = cus > 0 ? (uint32_t)cus : 1u;
+    int xccs = 0;  // XCDs; unknown (0) leaves the persistent form unavailable
+    if (hipDeviceGetAttribute(&xccs, hipDeviceAttributeNumberOfXccs, device) != hipSuccess) {
+        (void)hipGetLastError();
+        xccs = 0;
+    }
+    d.xcds = xccs > 0 ? (uint32_t)xccs : 0u;
     return NC_OK;
 }
 
@@ -532,7 +578,9 @@ int finish_plan(nc_plan* p) {
     const nc::HostTables& t = p->host;
     hipError_t e = nc::scratch_address(&p->scratch);  // this device's scratch array, looked up once per device
     if (e != hipSuccess) return hip_fail(e, "scratch address");
-    p->res_bytes = std::max((size_t)p->dev.w62.k * 8, (size_t)p->dev.w30.k * 4) * t.N;
+    // residue rows of the wider set, and a trailer for the persistent form's
+    // control block
+    p->res_bytes = std::max((size_t)p->dev.w62.k * 8, (size_t)p->dev.w30.k * 4) * t.N + nc::kPersistTrailer;
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_fail(e, "stream");
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
@@ -546,6 +594,8 @@ int finish_plan(nc_plan* p) {
         e = hipEventCreateWithFlags(&p->ev_piece[i], hipEventDisableTiming);
     if (e != hipSuccess) return hip_fail(e, "chunk streams");
     const size_t io = (size_t)t.N * t.L * sizeof(uint32_t);
+    NC_HIP(hipMalloc(&p->d_err, 256), "error word");
+    NC_HIP(hipMemset(p->d_err, 0, 256), "error word");
     NC_HIP(hipMalloc(&p->d_io, 3 * io), "staging");
     NC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_pin), 3 * io, hipHostMallocDefault), "pinned staging");
     void* hd = nullptr;  // the transfer kernels' view of it; without one, copies move it
@@ -915,6 +965,45 @@ int nc_plan_set_mfma(nc_plan* p, int mode) {
     return NC_OK;
 }
 
+int nc_plan_set_persist(nc_plan* p, int mode, uint32_t wait_bound_us) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (mode < -1 || mode > 1) return fail(NC_ERR_INVALID, "mode must be -1, 0 or 1");
+    p->persist_mode = mode;
+    p->persist_tmo = wait_bound_us ? (uint64_t)wait_bound_us * 100u : 100000000ull;  // 100 MHz ticks
+    return NC_OK;
+}
+
+int nc_plan_get_persist(const nc_plan* p, uint32_t batch, uint32_t* persistent) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    if (batch == 0) return fail(NC_ERR_INVALID, "batch must be >= 1");
+    const Choice c = choose(p, batch);
+    if (persistent) *persistent = (c.persist && nc::persist_available(p->dev, batch)) ? 1u : 0u;
+    return NC_OK;
+}
+
+int nc_plan_persist_status(nc_plan* p, void* stream, uint32_t* flags) {
+    if (!p) return fail(NC_ERR_INVALID, "null plan");
+    NC_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "status sync");
+    NC_HIP(hipEventSynchronize(p->ev_ws), "status sync");
+    uint32_t v = 0;
+    NC_HIP(hipMemcpy(&v, p->d_err, sizeof v, hipMemcpyDeviceToHost), "error word");
+    if (v) NC_HIP(hipMemset(p->d_err, 0, sizeof v), "error word");
+    if (flags) *flags = v;
+    return NC_OK;
+}
+
+int nc_persist_emulate(uint32_t N, uint32_t k, uint32_t batch, uint32_t xcds, uint32_t workgroups, uint64_t seed,
+                       double* makespan, double* wait_fraction) {
+    if (N < 1024 || N > 16384 || (N & (N - 1)))
+        return fail(NC_ERR_INVALID, "the persistent form covers N = 2^10 .. 2^14");
+    nc::PersistEmuStats st;
+    const int r = nc::persist_emulate((uint32_t)__builtin_ctz(N), k, batch, xcds, workgroups, seed, &st);
+    if (r) return fail(NC_ERR_STATE, "queue emulation failed check " + std::to_string(r));
+    if (makespan) *makespan = st.makespan;
+    if (wait_fraction) *wait_fraction = st.wait_fraction;
+    return NC_OK;
+}
+
 int nc_set_tune_cache(const char* path) {
     std::lock_guard<std::mutex> g(g_cache_mu);
     g_cache_path = path ? std::string(path) : std::string();
@@ -1017,6 +1106,20 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
                 if (e == hipSuccess) timed.push_back({median(runs), c});
             }
     }
+    // the persistent batched form (its own configuration: 30-bit set, both
+    // matrix-core paths, split 2), kept only if its timed runs drained every
+    // XCD's queue with no readiness wait reaching its bound
+    if (e == hipSuccess && p->persist_mode != 0 && !p->chunks_forced && nc::persist_available(p->dev, batch) &&
+        (p->mfma_mode < 0 || (uint32_t)p->mfma_mode == kMfmaAll)) {
+        Choice c;
+        c.width = 30;
+        c.mfma = kMfmaAll;
+        c.log_blocks = 2;
+        c.persist = 1;
+        std::vector<float> runs;
+        time_runs(c, 2, runs);
+        if (e == hipSuccess && persist_drained(p, batch, s)) timed.push_back({median(runs), c});
+    }
     // The quick pass above times each candidate twice; runs on one device
     // vary by several percent, as much as the leaders differ.  The three
     // fastest (and the fastest unchunked one, which graphs and the server
@@ -1029,13 +1132,20 @@ int nc_plan_autotune(nc_plan* p, uint32_t batch, void* stream, uint32_t* chosen_
         if (x.c.chunks == 1) {
             bool in = false;
             for (const Timed& y : top) in = in || (y.c.width == x.c.width && y.c.log_blocks == x.c.log_blocks &&
-                                                   y.c.mfma == x.c.mfma && y.c.chunks == 1);
+                                                   y.c.mfma == x.c.mfma && y.c.chunks == 1 &&
+                                                   y.c.persist == x.c.persist);
             if (!in) top.push_back(x);
             break;
         }
     std::vector<std::vector<float>> samples(top.size());
+    std::vector<bool> drained(top.size(), true);
     for (int round = 0; round < 3 && e == hipSuccess; ++round)
-        for (size_t i = 0; i < top.size() && e == hipSuccess; ++i) time_runs(top[i].c, 3, samples[i]);
+        for (size_t i = 0; i < top.size() && e == hipSuccess; ++i) {
+            time_runs(top[i].c, 3, samples[i]);
+            if (e == hipSuccess && top[i].c.persist && !persist_drained(p, batch, s)) drained[i] = false;
+        }
+    for (size_t i = 0; i < top.size(); ++i)
+        if (!drained[i]) samples[i].clear();
     int best = -1, best1 = -1;
     std::vector<float> med(top.size());
     for (size_t i = 0; i < top.size(); ++i) {

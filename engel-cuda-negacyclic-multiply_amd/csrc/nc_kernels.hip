@@ -184,53 +184,65 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 // of a group taking prime tiles [w T / WS, (w + 1) T / WS) of the T tiles,
 // so each wave's serial chain is WS times shorter; a work item then holds
 // 4 / WS chunks.
-template <int S1, int WS = 1>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void decompose30_mfma_kernel(
-    const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
-    uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int32_t* __restrict__ bias,
-    const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
-    uint32_t tiles, uint32_t nb, uint32_t tw_log) {
+struct DecompPrimeLds {
+    uint32_t p, mont_neg;
+};
+
+// LDS of the matrix-core decomposition: the int8 table, each prime's
+// constants, the row biases and, for the G = 2^S1 spread tiles of the fused
+// stages, psi^bitrev(x), x = 1 .. G - 1, of each prime.
+template <int G>
+struct DecompLds {
+    int8_t tab[kMfmaTiles * 32 * kMfmaRowStride];
+    DecompPrimeLds prime[kMfmaTiles * 8];
+    __attribute__((aligned(16))) int32_t bias[kMfmaTiles * 32];
+    Tw32 tw[kMfmaTiles * 8][G];
+};
+
+// Loads the decomposition's tables into s (every thread of a 256-thread
+// workgroup takes part; the caller synchronises before the first item).
+template <int S1>
+__device__ __forceinline__ void decompose30_mfma_tables(DecompLds<1 << S1>& s, const int8_t* __restrict__ tab,
+                                                        const int32_t* __restrict__ bias,
+                                                        const PrimeConst32* __restrict__ pc,
+                                                        const Tw32* __restrict__ twf, uint32_t k, uint32_t tiles,
+                                                        uint32_t tw_log) {
+    constexpr int G = 1 << S1;
+    for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
+        const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
+        *reinterpret_cast<nc_v4i*>(s.tab + row * kMfmaRowStride + chunk * 16) =
+            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
+    }
+    for (uint32_t r = threadIdx.x; r < tiles * 32; r += 256) s.bias[r] = bias[r];
+    for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256) {
+        s.prime[j] = j < k ? DecompPrimeLds{pc[j].p, pc[j].mont_neg} : DecompPrimeLds{1u, 0u};
+        if constexpr (S1 > 0) {
+#pragma unroll
+            for (int x = 1; x < G; ++x) s.tw[j][x] = j < k ? twf[((size_t)j << tw_log) + x] : Tw32{0u, 0u};
+        }
+    }
+}
+
+// One work item of the decomposition: 4 NG / WS chunks of 32 columns of one
+// polynomial (src: its N x L limbs; dst: its [k][N] residue rows), starting
+// at chunk chunk0.  No barrier inside.
+template <int S1, int WS>
+__device__ __forceinline__ void decompose30_mfma_item(const DecompLds<1 << S1>& s, const uint32_t* __restrict__ src,
+                                                      uint32_t* __restrict__ dst, uint32_t chunk0, uint32_t N,
+                                                      uint32_t L, uint32_t k, uint32_t tiles) {
     constexpr int G = 1 << S1;                 // spread tiles per column group
     constexpr int NG = kMfmaCoeffTiles / G;    // column groups per wave
     static_assert(NG >= 1, "at most 4 spread tiles");
-    __shared__ int8_t lds_tab[kMfmaTiles * 32 * kMfmaRowStride];
-    struct PrimeLds {
-        uint32_t p, mont_neg;
-    };
-    __shared__ PrimeLds lds_prime[kMfmaTiles * 8];
-    __shared__ __attribute__((aligned(16))) int32_t lds_bias[kMfmaTiles * 32];
-    __shared__ Tw32 lds_tw[kMfmaTiles * 8][G];  // psi^bitrev(x), x = 1 .. G - 1, of each prime
-    for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
-        const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
-        *reinterpret_cast<nc_v4i*>(lds_tab + row * kMfmaRowStride + chunk * 16) =
-            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
-    }
-    for (uint32_t r = threadIdx.x; r < tiles * 32; r += 256) lds_bias[r] = bias[r];
-    for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256) {
-        lds_prime[j] = j < k ? PrimeLds{pc[j].p, pc[j].mont_neg} : PrimeLds{1u, 0u};
-        if constexpr (S1 > 0) {
-#pragma unroll
-            for (int x = 1; x < G; ++x) lds_tw[j][x] = j < k ? twf[((size_t)j << tw_log) + x] : Tw32{0u, 0u};
-        }
-    }
-    __syncthreads();
+    static_assert(WS == 1 || WS == 2 || WS == 4, "waves per chunk");
+    const int8_t* lds_tab = s.tab;
+    const DecompPrimeLds* lds_prime = s.prime;
+    const int32_t* lds_bias = s.bias;
     const uint32_t span = N >> S1;                      // columns of one spread tile group
     const uint32_t chunks = span / 32;                  // 32-column chunks per polynomial
-    static_assert(WS == 1 || WS == 2 || WS == 4, "waves per chunk");
-    constexpr uint32_t kChunksPerItem = 4 * NG / WS;
-    const uint32_t wgs_per_poly = (chunks + kChunksPerItem - 1) / kChunksPerItem;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 31u, half = lane >> 5;
     const uint32_t nbytes = 4 * L;
-    // persistent: the tables above are loaded once per workgroup and serve
-    // every work item it takes (work item = 4 NG / WS chunks of one polynomial)
-    for (uint32_t item = blockIdx.x; item < wgs_per_poly * (na + nb); item += gridDim.x) {
-    const uint32_t poly = item / wgs_per_poly;
-    const uint32_t chunk0 = (item - poly * wgs_per_poly) * kChunksPerItem;
-    const bool is_a = poly < na;
-    const uint32_t pi = is_a ? poly : poly - na;
-    const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
-    uint32_t* dst = (is_a ? res_a : res_b) + (size_t)pi * k * N;
+#pragma unroll 1
     for (int g = 0; g < NG; ++g) {
         const uint32_t ch = chunk0 + (wave / WS) * NG + g;
         if (ch >= chunks) break;  // wave-uniform; no barrier in the item loop
@@ -281,7 +293,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                                                                 bf[e][ks], acc, 0, 0, 0);
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
-                    const PrimeLds cj = lds_prime[t * 8 + 2 * h + half];
+                    const DecompPrimeLds cj = lds_prime[t * 8 + 2 * h + half];
                     const int32_t lo = acc[4 * h] + (int32_t)((uint32_t)acc[4 * h + 1] << 8);
                     const int32_t hi = acc[4 * h + 2] + (int32_t)((uint32_t)acc[4 * h + 3] << 8);
                     const uint64_t V = (uint64_t)((int64_t)hi * two16 + (int64_t)lo);
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
                 if constexpr (S1 > 0) {
                     const uint32_t p = lds_prime[j].p;
                     ct_pass<W30, S1, G>(v[h], c0, (int)(31 - __builtin_clz(span)), (int)(31 - __builtin_clz(N)),
-                                        &lds_tw[j][0], p, 2 * p);
+                                        &s.tw[j][0], p, 2 * p);
                 }
                 if (j < k) {
 #pragma unroll
@@ -307,6 +319,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void d
             }
         }
     }
+}
+
+template <int S1, int WS = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void decompose30_mfma_kernel(
+    const uint32_t* __restrict__ a, uint32_t* __restrict__ res_a, uint32_t na, const uint32_t* __restrict__ b,
+    uint32_t* __restrict__ res_b, const int8_t* __restrict__ tab, const int32_t* __restrict__ bias,
+    const PrimeConst32* __restrict__ pc, const Tw32* __restrict__ twf, uint32_t N, uint32_t L, uint32_t k,
+    uint32_t tiles, uint32_t nb, uint32_t tw_log) {
+    constexpr int NG = kMfmaCoeffTiles / (1 << S1);
+    __shared__ DecompLds<1 << S1> s;
+    decompose30_mfma_tables<S1>(s, tab, bias, pc, twf, k, tiles, tw_log);
+    __syncthreads();
+    const uint32_t chunks = (N >> S1) / 32;  // 32-column chunks per polynomial
+    constexpr uint32_t kChunksPerItem = 4 * NG / WS;
+    const uint32_t wgs_per_poly = (chunks + kChunksPerItem - 1) / kChunksPerItem;
+    // persistent: the tables above are loaded once per workgroup and serve
+    // every work item it takes (work item = 4 NG / WS chunks of one polynomial)
+    for (uint32_t item = blockIdx.x; item < wgs_per_poly * (na + nb); item += gridDim.x) {
+        const uint32_t poly = item / wgs_per_poly;
+        const uint32_t chunk0 = (item - poly * wgs_per_poly) * kChunksPerItem;
+        const bool is_a = poly < na;
+        const uint32_t pi = is_a ? poly : poly - na;
+        const uint32_t* src = (is_a ? a : b) + (size_t)pi * N * L;
+        uint32_t* dst = (is_a ? res_a : res_b) + (size_t)pi * k * N;
+        decompose30_mfma_item<S1, WS>(s, src, dst, chunk0, N, L, k, tiles);
     }
 }
 
@@ -693,41 +730,24 @@ __device__ __forceinline__ void inverse30_from(uint32_t (&v)[16], uint32_t* lds,
     }
 }
 
+// One 2^LOGB block of one prime row: ra_blk / rb_blk point at the block in
+// a's and b's rows (the product overwrites ra_blk), gblk is the block's first
+// row index, tf / ti the prime's twiddle rows.  tid: the thread's index in
+// its (sub-)block; lds2: the sub-block's padded region.  Every thread of the
+// workgroup calls this (barriers inside); an inactive sub-block computes and
+// stores nothing it is not given to store.
 template <int LOGB>
-__global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
-    uint32_t* __restrict__ ra, const uint32_t* __restrict__ rb, const Tw32* __restrict__ twf,
-    const Tw32* __restrict__ twi, const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t k, uint32_t nprod,
-    uint32_t tw_log) {
+__device__ __forceinline__ void ntt_inner30_block(uint2* lds2, uint32_t tid, uint32_t* __restrict__ ra_blk,
+                                                  const uint32_t* __restrict__ rb_blk, uint32_t gblk,
+                                                  const PrimeConst32& c, const Tw32* tf, const Tw32* ti,
+                                                  uint32_t log_n, bool active, bool final_out) {
     using S = InnerShape<LOGB>;
     using T = uint32_t;
     constexpr int LAST = S::NP - 1;
-    __shared__ uint2 lds_all[S::BPW * S::PADB];
-    const uint32_t sub = threadIdx.x / S::TPB;
-    const uint32_t tid = threadIdx.x % S::TPB;
-    uint2* lds2 = lds_all + sub * S::PADB;
     T* lds1 = reinterpret_cast<T*>(lds2);  // the inverse transform reuses the region for one operand
-    const uint32_t N = 1u << log_n;
-    const uint32_t nblk_log = log_n - LOGB;
-    const uint32_t total = k * nprod << nblk_log;
-    uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * S::BPW + sub;
-    if constexpr (S::TPB >= 64) wid = uniform(wid);
-    const bool active = wid < total;
-    // an idle sub-block works on block 0 (its loads stay in bounds and
-    // unconditional, so none of them needs a branch) for uniform barriers,
-    // and stores nothing
-    if (!active) wid = 0;
-    const uint32_t blk = wid & ((1u << nblk_log) - 1);
-    const uint32_t row = wid >> nblk_log;  // = j * nprod + prod
-    const uint32_t j = row / nprod, prod = row - j * nprod;
-    const size_t off = ((size_t)prod * k + j) * N + ((size_t)blk << LOGB);
-    const uint32_t gblk = blk << LOGB;
-    const PrimeConst32 c = pcs[j];
-    const Tw32* tf = twf + ((size_t)j << tw_log);
-    const Tw32* ti = twi + ((size_t)j << tw_log);
-    const bool final_out = (nblk_log == 0);
     using P0 = PassMap<LOGB, 0>;  // lane-contiguous: x = tid + e TPB
-    T* ga0 = ra + off + P0::base(tid);
-    const T* gb0 = rb + off + P0::base(tid);
+    T* ga0 = ra_blk + P0::base(tid);
+    const T* gb0 = rb_blk + P0::base(tid);
     T va[16], vb[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -751,6 +771,35 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
     if (!active) return;
 #pragma unroll
     for (int e = 0; e < 16; ++e) ga0[P0::delta(e)] = final_out ? W30::sub_if_ge(va[e], c.p) : va[e];
+}
+
+template <int LOGB>
+__global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner30_kernel(
+    uint32_t* __restrict__ ra, const uint32_t* __restrict__ rb, const Tw32* __restrict__ twf,
+    const Tw32* __restrict__ twi, const PrimeConst32* __restrict__ pcs, uint32_t log_n, uint32_t k, uint32_t nprod,
+    uint32_t tw_log) {
+    using S = InnerShape<LOGB>;
+    __shared__ uint2 lds_all[S::BPW * S::PADB];
+    const uint32_t sub = threadIdx.x / S::TPB;
+    const uint32_t tid = threadIdx.x % S::TPB;
+    const uint32_t N = 1u << log_n;
+    const uint32_t nblk_log = log_n - LOGB;
+    const uint32_t total = k * nprod << nblk_log;
+    uint32_t wid = xcd_remap(blockIdx.x, gridDim.x) * S::BPW + sub;
+    if constexpr (S::TPB >= 64) wid = uniform(wid);
+    const bool active = wid < total;
+    // an idle sub-block works on block 0 (its loads stay in bounds and
+    // unconditional, so none of them needs a branch) for uniform barriers,
+    // and stores nothing
+    if (!active) wid = 0;
+    const uint32_t blk = wid & ((1u << nblk_log) - 1);
+    const uint32_t row = wid >> nblk_log;  // = j * nprod + prod
+    const uint32_t j = row / nprod, prod = row - j * nprod;
+    const size_t off = ((size_t)prod * k + j) * N + ((size_t)blk << LOGB);
+    const PrimeConst32 c = pcs[j];
+    ntt_inner30_block<LOGB>(lds_all + sub * S::PADB, tid, ra + off, rb + off, blk << LOGB, c,
+                            twf + ((size_t)j << tw_log), twi + ((size_t)j << tw_log), log_n, active,
+                            nblk_log == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,49 +1217,74 @@ struct CrtInvLds {
     uint32_t p, w, ws, wf, wfs, sadd, pad0, pad1;
 };
 
-template <int RT, int KCT, int LT, int INV = 0>
-__device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res, uint32_t* __restrict__ out,
-                                                const int8_t* __restrict__ tab, const int64_t* __restrict__ corr,
-                                                const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
-                                                uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n,
-                                                uint32_t nprod, const PrimeConst32* __restrict__ pcs = nullptr,
-                                                const Tw32* __restrict__ twi = nullptr, uint32_t tw_log = 0) {
-    static_assert(INV == 0 || INV == 2, "fused inverse stages: 0 or 2");
-    constexpr int LF = 8 * RT - 4;                         // fraction pseudo-limbs LF, LF + 1
-    constexpr int LQ = LF < kMaxLimbs ? LF : kMaxLimbs;  // q limbs (zero beyond L)
+// LDS of the matrix-core CRT: the int8 table (row pitch PITCH bytes), the
+// output staging of the 4 waves (STAGE lanes of LW words each: 64, or 32 when
+// a wave stages one lane half at a time) and, for the fused inverse stages,
+// NINV per-lane-half prime constants.
+template <int RT, int PITCH, int LW, int STAGE, int NINV>
+struct CrtLds {
+    int8_t tab[RT * 32 * PITCH];
+    __attribute__((aligned(16))) uint32_t out[4 * STAGE * LW];
+    __attribute__((aligned(16))) CrtInvLds inv[NINV];
+};
+
+template <int RT, int KCT, int LT, int INV, int PITCH = kCrtRowStride, int STAGE = 64>
+using CrtLdsOf = CrtLds<RT, PITCH, (LT > 0 ? LT : kMaxLimbs), STAGE,
+                        (INV ? 2 * 8 * (KCT > 0 ? KCT : kCrtMaxKC) : 1)>;
+
+// Loads the CRT's tables into s (every thread of a 256-thread workgroup
+// takes part; the caller synchronises before the first group).
+template <int RT, int KCT, int LT, int INV, int PITCH, class Lds>
+__device__ __forceinline__ void crt30_mfma_tables(Lds& s, const int8_t* __restrict__ tab, uint32_t k, uint32_t kc_n,
+                                                  const PrimeConst32* __restrict__ pcs, const Tw32* __restrict__ twi,
+                                                  uint32_t tw_log) {
     constexpr int KCM = KCT > 0 ? KCT : kCrtMaxKC;
-    __shared__ int8_t lds_tab[RT * 32 * kCrtRowStride];
-    __shared__ __attribute__((aligned(16))) uint32_t lds_out[4 * 64 * (LT > 0 ? LT : kMaxLimbs)];
     if (KCT > 0) kc_n = KCT;
-    if (LT > 0) L = LT;
     const uint32_t kbytes = 32 * kc_n;
     const uint32_t chunks = kbytes / 16;
     for (uint32_t w = threadIdx.x; w < RT * 32 * chunks; w += 256) {
         const uint32_t row = w / chunks, ch = w - row * chunks;
-        *reinterpret_cast<nc_v4i*>(lds_tab + row * kCrtRowStride + ch * 16) =
+        *reinterpret_cast<nc_v4i*>(s.tab + row * PITCH + ch * 16) =
             *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kbytes + ch * 16);
     }
     // entries 8 kc + 4 h + e for every chunk; rows >= k repeat row k - 1
-    __shared__ __attribute__((aligned(16))) CrtInvLds lds_inv[INV ? 2 * 8 * KCM : 1];
     if constexpr (INV > 0) {
         for (uint32_t r2 = threadIdx.x; r2 < 16u * KCM; r2 += 256) {
             const uint32_t r = r2 >> 1, u = r2 & 1u;
             const uint32_t j = r < k ? r : k - 1;
             const PrimeConst32 c = pcs[j];
             const Tw32 t = twi[((size_t)j << tw_log) + 2 + u];
-            lds_inv[r2] = u ? CrtInvLds{c.p, t.w, t.ws, c.wr, c.wr_s, 2 * c.p + 1, 0u, 0u}
-                            : CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, 0u, 0u, 0u};
+            s.inv[r2] = u ? CrtInvLds{c.p, t.w, t.ws, c.wr, c.wr_s, 2 * c.p + 1, 0u, 0u}
+                          : CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, 0u, 0u, 0u};
         }
     }
-    __syncthreads();
+}
+
+// One wave's group of 64 coefficients starting at g0 (< total = nprod N; a
+// multiple of 64): the contraction, the Montgomery finish and the store to
+// out.  No workgroup barrier inside.
+template <int RT, int KCT, int LT, int INV, int PITCH, int STAGE, class Lds>
+__device__ __forceinline__ void crt30_mfma_group(const Lds& s, uint32_t g0, const uint32_t* __restrict__ res,
+                                                 uint32_t* __restrict__ out, const int64_t* __restrict__ corr,
+                                                 const CrtMfmaConst* __restrict__ mc,
+                                                 const QConst* __restrict__ qc, uint32_t log_n, uint32_t L,
+                                                 uint32_t k, uint32_t kc_n, uint32_t total, uint32_t q_inv) {
+    static_assert(INV == 0 || INV == 2, "fused inverse stages: 0 or 2");
+    static_assert(STAGE == 64 || STAGE == 32, "staging: whole wave or one lane half");
+    constexpr int LF = 8 * RT - 4;                         // fraction pseudo-limbs LF, LF + 1
+    constexpr int LQ = LF < kMaxLimbs ? LF : kMaxLimbs;  // q limbs (zero beyond L)
+    constexpr int KCM = KCT > 0 ? KCT : kCrtMaxKC;
+    const int8_t* lds_tab = s.tab;
+    const CrtInvLds* lds_inv = s.inv;
+    if (KCT > 0) kc_n = KCT;
+    if (LT > 0) L = LT;
+    const uint32_t kbytes = 32 * kc_n;
     // kc_n chunks of 8 primes: k > 8 (KCT - 1), so only the last chunk has rows >= k
     if (KCT > 0 && k <= 8u * (KCT - 1)) __builtin_unreachable();
     const uint32_t N = 1u << log_n;
-    const uint32_t total = nprod << log_n;  // a multiple of 32 (N >= 32)
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t col = lane & 31u, half = lane >> 5;
     const int64_t sb_off = 128ll * kbytes;
-    const uint32_t q_inv = mc->q_inv;
     // B operands of both tiles of a wave's 64 coefficients starting at g0,
     // every load issued before the first use; rows j >= k read row k - 1 and
     // are replaced by 0 at use
@@ -1254,127 +1328,125 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
             }
         }
     };
-    for (uint32_t grp = blockIdx.x; grp * 256u < total; grp += gridDim.x) {
-        const uint32_t g0 = grp * 256u + wave * 64u;
-        if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
-        load_group(g0);
-        if constexpr (INV > 0) {
-            // last two inverse stages, residues left in [0, 2p) (rows >= k are
-            // replaced by 0 at use; their constants are row k - 1's).  The
-            // memory clobber keeps the per-prime constants from being hoisted
-            // out of the group loop (32 primes x 9 words would not fit).
-            asm volatile("" ::: "memory");
-            const uint32_t upper = (col >> 4) & 1u;
-            const uint32_t m = 0u - upper;  // all ones in the upper half
+    load_group(g0);
+    if constexpr (INV > 0) {
+        // last two inverse stages, residues left in [0, 2p) (rows >= k are
+        // replaced by 0 at use; their constants are row k - 1's).  The
+        // memory clobber keeps the per-prime constants from being hoisted
+        // out of the group loop (32 primes x 9 words would not fit).
+        asm volatile("" ::: "memory");
+        const uint32_t upper = (col >> 4) & 1u;
+        const uint32_t m = 0u - upper;  // all ones in the upper half
 #pragma unroll
-            for (int kc = 0; kc < KCM; ++kc) {
-                if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
+        for (int kc = 0; kc < KCM; ++kc) {
+            if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const CrtInvLds ci = lds_inv[2 * (8 * kc + 4 * half + e) + upper];
-                    const uint32_t p = ci.p, p2 = 2 * p;
-                    // span N/4: tile 0 (x) with tile 1 (x + N/4)
-                    uint32_t a = (uint32_t)bf[0][kc][e], b = (uint32_t)bf[1][kc][e];
-                    Bfly<W30>::gs(a, b, Tw32{ci.w, ci.ws}, p, p2);
-                    // span N/2 (stage 0): lane c pairs with lane c ^ 16; the
-                    // lower half forms a + pa, the upper pa - a + 2p, both as
-                    // pa + (a ^ m) + sadd
-                    const uint32_t pa = (uint32_t)__shfl_xor((int)a, 16);
-                    const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
-                    const uint32_t sa = pa + (a ^ m) + ci.sadd;
-                    const uint32_t sb = pb + (b ^ m) + ci.sadd;
-                    bf[0][kc][e] = (int)Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p);
-                    bf[1][kc][e] = (int)Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p);
-                }
+            for (int e = 0; e < 4; ++e) {
+                const CrtInvLds ci = lds_inv[2 * (8 * kc + 4 * half + e) + upper];
+                const uint32_t p = ci.p, p2 = 2 * p;
+                // span N/4: tile 0 (x) with tile 1 (x + N/4)
+                uint32_t a = (uint32_t)bf[0][kc][e], b = (uint32_t)bf[1][kc][e];
+                Bfly<W30>::gs(a, b, Tw32{ci.w, ci.ws}, p, p2);
+                // span N/2 (stage 0): lane c pairs with lane c ^ 16; the
+                // lower half forms a + pa, the upper pa - a + 2p, both as
+                // pa + (a ^ m) + sadd
+                const uint32_t pa = (uint32_t)__shfl_xor((int)a, 16);
+                const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
+                const uint32_t sa = pa + (a ^ m) + ci.sadd;
+                const uint32_t sb = pb + (b ^ m) + ci.sadd;
+                bf[0][kc][e] = (int)Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p);
+                bf[1][kc][e] = (int)Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p);
             }
         }
-        // One row tile: both tiles' contractions (the A fragment serves both),
-        // then the exchange.  Lane (c, 0) keeps tile 0's even pseudo-limbs and
-        // receives its odd ones; lane (c, 1) receives tile 1's even ones and
-        // keeps the odd.  |D| <= Kdim 2^14 < 2^23: byte pairs fit int32.
-        auto row_tile = [&](int rt, int32_t (&elo)[4], int32_t (&ehi)[4], int32_t (&olo)[4], int32_t (&ohi)[4],
-                            int32_t* dones) {
-            nc_v16i acc0 = {}, acc1 = {};
-            const int8_t* arow = lds_tab + (rt * 32 + col) * kCrtRowStride + 16 * half;
+    }
+    // One row tile: both tiles' contractions (the A fragment serves both),
+    // then the exchange.  Lane (c, 0) keeps tile 0's even pseudo-limbs and
+    // receives its odd ones; lane (c, 1) receives tile 1's even ones and
+    // keeps the odd.  |D| <= Kdim 2^14 < 2^23: byte pairs fit int32.
+    auto row_tile = [&](int rt, int32_t (&elo)[4], int32_t (&ehi)[4], int32_t (&olo)[4], int32_t (&ohi)[4],
+                        int32_t* dones) {
+        nc_v16i acc0 = {}, acc1 = {};
+        const int8_t* arow = lds_tab + (rt * 32 + col) * PITCH + 16 * half;
 #pragma unroll
-            for (int kc = 0; kc < KCM; ++kc) {
-                if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
-                nc_v4i b0, b1;
+        for (int kc = 0; kc < KCM; ++kc) {
+            if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
+            nc_v4i b0, b1;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const bool live = 8 * kc + 4 * half + e < k;
-                    b0[e] = live ? bf[0][kc][e] ^ (int)0x80808080 : (int)0x80808080;
-                    b1[e] = live ? bf[1][kc][e] ^ (int)0x80808080 : (int)0x80808080;
-                }
-                const nc_v4i a = *reinterpret_cast<const nc_v4i*>(arow + 32 * kc);
-                acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b1, acc1, 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+                const bool live = 8 * kc + 4 * half + e < k;
+                b0[e] = live ? bf[0][kc][e] ^ (int)0x80808080 : (int)0x80808080;
+                b1[e] = live ? bf[1][kc][e] ^ (int)0x80808080 : (int)0x80808080;
             }
-            if (dones) {  // ones row of pseudo-limb LF + 2 + h, before the swap: own tile
-                dones[0] = acc0[12];
-                dones[1] = acc1[12];
-            }
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t lo0 = (uint32_t)(acc0[4 * m] + (int32_t)((uint32_t)acc0[4 * m + 1] << 8));
-                const uint32_t hi0 = (uint32_t)(acc0[4 * m + 2] + (int32_t)((uint32_t)acc0[4 * m + 3] << 8));
-                const uint32_t lo1 = (uint32_t)(acc1[4 * m] + (int32_t)((uint32_t)acc1[4 * m + 1] << 8));
-                const uint32_t hi1 = (uint32_t)(acc1[4 * m + 2] + (int32_t)((uint32_t)acc1[4 * m + 3] << 8));
-                const auto sl = __builtin_amdgcn_permlane32_swap(lo0, lo1, false, false);
-                const auto sh = __builtin_amdgcn_permlane32_swap(hi0, hi1, false, false);
-                elo[m] = (int32_t)sl[0];
-                olo[m] = (int32_t)sl[1];
-                ehi[m] = (int32_t)sh[0];
-                ohi[m] = (int32_t)sh[1];
-            }
-        };
-        // the last row tile first: it carries the byte-sum row, so every later
-        // tile's limbs can be carried into 32-bit limbs as soon as they exist
-        int32_t dones[2];
-        int32_t lelo[4], lehi[4], lolo[4], lohi[4];
-        row_tile(RT - 1, lelo, lehi, lolo, lohi, dones);
-        const int64_t sb = (int64_t)(half ? dones[1] : dones[0]) + sb_off;
-        const int64_t sbterm = 128 * sb * 0x01010101ll;
-        auto ysum = [&](int lam, int32_t lo, int32_t hi) -> int64_t {
-            return (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sbterm + corr[lam];
-        };
-        constexpr int NA = LQ + 4;
-        uint32_t x[NA];
-        int64_t carry = 0;
-#pragma unroll
-        for (int rt = 0; rt < RT - 1; ++rt) {
-            int32_t elo[4], ehi[4], olo[4], ohi[4];
-            row_tile(rt, elo, ehi, olo, ohi, nullptr);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int lam = 8 * rt + i;
-                if (lam >= LQ) break;
-                const int m = i / 2;
-                const int64_t t = ysum(lam, (i & 1) ? olo[m] : elo[m], (i & 1) ? ohi[m] : ehi[m]) + carry;
-                x[lam] = (uint32_t)t;
-                carry = t >> 32;
-            }
+            const nc_v4i a = *reinterpret_cast<const nc_v4i*>(arow + 32 * kc);
+            acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b1, acc1, 0, 0, 0);
         }
-        auto last_sum = [&](int lam) -> int64_t {  // pseudo-limb of the last row tile
-            const int i = lam - 8 * (RT - 1), m = i / 2;
-            return ysum(lam, (i & 1) ? lolo[m] : lelo[m], (i & 1) ? lohi[m] : lehi[m]);
-        };
+        if (dones) {  // ones row of pseudo-limb LF + 2 + h, before the swap: own tile
+            dones[0] = acc0[12];
+            dones[1] = acc1[12];
+        }
 #pragma unroll
-        for (int lam = 8 * (RT - 1); lam < LQ; ++lam) {
-            const int64_t t = last_sum(lam) + carry;
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t lo0 = (uint32_t)(acc0[4 * m] + (int32_t)((uint32_t)acc0[4 * m + 1] << 8));
+            const uint32_t hi0 = (uint32_t)(acc0[4 * m + 2] + (int32_t)((uint32_t)acc0[4 * m + 3] << 8));
+            const uint32_t lo1 = (uint32_t)(acc1[4 * m] + (int32_t)((uint32_t)acc1[4 * m + 1] << 8));
+            const uint32_t hi1 = (uint32_t)(acc1[4 * m + 2] + (int32_t)((uint32_t)acc1[4 * m + 3] << 8));
+            const auto sl = __builtin_amdgcn_permlane32_swap(lo0, lo1, false, false);
+            const auto sh = __builtin_amdgcn_permlane32_swap(hi0, hi1, false, false);
+            elo[m] = (int32_t)sl[0];
+            olo[m] = (int32_t)sl[1];
+            ehi[m] = (int32_t)sh[0];
+            ohi[m] = (int32_t)sh[1];
+        }
+    };
+    // the last row tile first: it carries the byte-sum row, so every later
+    // tile's limbs can be carried into 32-bit limbs as soon as they exist
+    int32_t dones[2];
+    int32_t lelo[4], lehi[4], lolo[4], lohi[4];
+    row_tile(RT - 1, lelo, lehi, lolo, lohi, dones);
+    const int64_t sb = (int64_t)(half ? dones[1] : dones[0]) + sb_off;
+    const int64_t sbterm = 128 * sb * 0x01010101ll;
+    auto ysum = [&](int lam, int32_t lo, int32_t hi) -> int64_t {
+        return (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sbterm + corr[lam];
+    };
+    constexpr int NA = LQ + 4;
+    uint32_t x[NA];
+    int64_t carry = 0;
+#pragma unroll
+    for (int rt = 0; rt < RT - 1; ++rt) {
+        int32_t elo[4], ehi[4], olo[4], ohi[4];
+        row_tile(rt, elo, ehi, olo, ohi, nullptr);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int lam = 8 * rt + i;
+            if (lam >= LQ) break;
+            const int m = i / 2;
+            const int64_t t = ysum(lam, (i & 1) ? olo[m] : elo[m], (i & 1) ? ohi[m] : ehi[m]) + carry;
             x[lam] = (uint32_t)t;
             carry = t >> 32;
         }
-        x[LQ] = (uint32_t)carry;  // the sum is non-negative and below 2^(32 L + 17)
+    }
+    auto last_sum = [&](int lam) -> int64_t {  // pseudo-limb of the last row tile
+        const int i = lam - 8 * (RT - 1), m = i / 2;
+        return ysum(lam, (i & 1) ? lolo[m] : lelo[m], (i & 1) ? lohi[m] : lehi[m]);
+    };
 #pragma unroll
-        for (int l = LQ + 1; l < NA; ++l) x[l] = 0;
-        const uint64_t F = (uint64_t)last_sum(LF) + ((uint64_t)last_sum(LF + 1) << 32);
-        const uint32_t K = (uint32_t)(F >> 56) + (uint32_t)((F >> 55) & 1u);
-        mac_limbs<NA>(x, mc->neg_m_r, LQ, K, 0);
-        redc96<LQ, NA>(x, qc, q_inv);
-        const uint32_t* r = x + 3;
-        // lane (c, h) holds coefficient g0 + 32 h + c = g0 + lane
-        uint32_t* st = lds_out + wave * 64u * L;
+    for (int lam = 8 * (RT - 1); lam < LQ; ++lam) {
+        const int64_t t = last_sum(lam) + carry;
+        x[lam] = (uint32_t)t;
+        carry = t >> 32;
+    }
+    x[LQ] = (uint32_t)carry;  // the sum is non-negative and below 2^(32 L + 17)
+#pragma unroll
+    for (int l = LQ + 1; l < NA; ++l) x[l] = 0;
+    const uint64_t F = (uint64_t)last_sum(LF) + ((uint64_t)last_sum(LF + 1) << 32);
+    const uint32_t K = (uint32_t)(F >> 56) + (uint32_t)((F >> 55) & 1u);
+    mac_limbs<NA>(x, mc->neg_m_r, LQ, K, 0);
+    redc96<LQ, NA>(x, qc, q_inv);
+    const uint32_t* r = x + 3;
+    // lane (c, h) holds coefficient g0 + 32 h + c = g0 + lane
+    uint32_t* st = const_cast<uint32_t*>(s.out) + wave * (uint32_t)STAGE * L;
+    if constexpr (STAGE == 64) {
 #pragma unroll
         for (int l = 0; l < LQ; ++l)
             if ((uint32_t)l < L) st[lane * L + l] = r[l];
@@ -1398,6 +1470,54 @@ __device__ __forceinline__ void crt30_mfma_body(const uint32_t* __restrict__ res
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        // one lane half at a time: the staging holds 32 coefficients, the
+        // words of lanes 32 hh .. 32 hh + 31 (32 L of the wave's 64 L)
+        static_assert(INV > 0, "half staging is built for the fused form");
+        const uint32_t x0 = inv_x0(g0), run_words = 16u * L;  // a multiple of 4
+        uint32_t* dst = out + (size_t)(g0 >> log_n) * N * L;
+#pragma unroll
+        for (uint32_t hh = 0; hh < 2; ++hh) {
+            if (half == hh) {
+#pragma unroll
+                for (int l = 0; l < LQ; ++l)
+                    if ((uint32_t)l < L) st[col * L + l] = r[l];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t w = lane * 4u; w < 32u * L; w += 256u) {
+                const uint32_t wg = hh * 32u * L + w;  // word of the wave's 64 L
+                const uint32_t r = wg / run_words, within = wg - r * run_words;
+                const uint32_t start = x0 + (r & 1u) * (N >> 1) + (r >> 1) * (N >> 2);
+                *reinterpret_cast<nc_v4i*>(dst + (size_t)start * L + within) =
+                    *reinterpret_cast<const nc_v4i*>(st + w);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+// A CRT launch: tables to LDS once, then 256-coefficient groups (one wave
+// each quarter) strided over the grid.
+template <int RT, int KCT, int LT, int INV = 0>
+__device__ __forceinline__ void crt30_mfma_run(const uint32_t* __restrict__ res, uint32_t* __restrict__ out,
+                                               const int8_t* __restrict__ tab, const int64_t* __restrict__ corr,
+                                               const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
+                                               uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod,
+                                               const PrimeConst32* __restrict__ pcs = nullptr,
+                                               const Tw32* __restrict__ twi = nullptr, uint32_t tw_log = 0) {
+    __shared__ CrtLdsOf<RT, KCT, LT, INV> s;
+    crt30_mfma_tables<RT, KCT, LT, INV, kCrtRowStride>(s, tab, k, kc_n, pcs, twi, tw_log);
+    __syncthreads();
+    const uint32_t total = nprod << log_n;  // a multiple of 32 (N >= 32)
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t q_inv = mc->q_inv;
+    for (uint32_t grp = blockIdx.x; grp * 256u < total; grp += gridDim.x) {
+        const uint32_t g0 = grp * 256u + wave * 64u;
+        if (g0 >= total) continue;  // wave-uniform; no barrier in this loop
+        crt30_mfma_group<RT, KCT, LT, INV, kCrtRowStride, 64>(s, g0, res, out, corr, mc, qc, log_n, L, k, kc_n, total,
+                                                               q_inv);
     }
 }
 
@@ -1409,7 +1529,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
     uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod, const PrimeConst32* __restrict__ pcs,
     const Tw32* __restrict__ twi, uint32_t tw_log) {
-    crt30_mfma_body<RT, KCT, LT, INV>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod, pcs, twi, tw_log);
+    crt30_mfma_run<RT, KCT, LT, INV>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod, pcs, twi, tw_log);
 }
 
 template <int RT>
@@ -1417,7 +1537,184 @@ __global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
     const uint32_t* __restrict__ res, uint32_t* __restrict__ out, const int8_t* __restrict__ tab,
     const int64_t* __restrict__ corr, const CrtMfmaConst* __restrict__ mc, const QConst* __restrict__ qc,
     uint32_t log_n, uint32_t L, uint32_t k, uint32_t kc_n, uint32_t nprod) {
-    crt30_mfma_body<RT, 0, 0>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod);
+    crt30_mfma_run<RT, 0, 0>(res, out, tab, corr, mc, qc, log_n, L, k, kc_n, nprod);
+}
+
+// ===========================================================================
+// Persistent batched form (nc_persist.hpp): one launch per batch, every XCD
+// draining its own queue of decomposition, NTT and CRT items, so that the
+// three stages of different products share every compute unit.  Shape: 30-bit
+// primes, both matrix-core paths, outer split 2 (the decomposition runs the
+// first two forward stages, the CRT the last two inverse stages), L = 28 and
+// 57..64 primes, blocks of N / 4 = 2^LOGB <= 2^12 values.
+//
+// Hand-off between items: all items of a product run on one XCD (the hardware
+// register XCC_ID names it), whose compute units share one L2.  A producer's
+// stores are complete in that L2 once its waves pass s_waitcnt vmcnt(0); the
+// workgroup then bumps its product's counter with an atomic add.  A consumer
+// polls the counter with atomic read-modify-writes (which meet the adds at the
+// same place), then invalidates its compute unit's L1 (buffer_inv sc0) before
+// it loads.  Each wait is bounded in time (s_memrealtime, 100 MHz) and by an
+// iteration cap: on expiry the workgroup sets the XCD's abort flag and error
+// bits and the plan's error word, and every workgroup of that XCD stops, so
+// the launch always ends.
+// ===========================================================================
+constexpr int kPersistCrtPitch = 8 * 32 + 16;  // CRT table pitch for 8 chunks of 8 primes (16-byte padded)
+constexpr uint32_t kPersistWaitCap = 1u << 22;  // polls per wait at most (the time bound comes first)
+
+struct PersistArgs {
+    const uint32_t* a;  // [batch][N][L] operands
+    const uint32_t* b;
+    uint32_t* c;        // [batch][N][L] products
+    uint32_t* ra;       // [batch][k][N] residue rows of a (the product's rows overwrite them)
+    uint32_t* rb;
+    uint32_t* ctl;      // nxcd control blocks of xcd_words words (nc_persist.hpp)
+    uint32_t* sticky;   // the plan's error word (bits accumulate until the host reads them)
+    DevSet30 w;
+    uint32_t N, log_n, L, batch, nxcd, xcd_words;
+    uint32_t R, nD, nT, nC;
+    uint32_t lag;         // rounds between a product's D, T and C items
+    uint32_t d_per_poly;  // D items per operand
+    uint64_t tmo;       // bound of one readiness wait, s_memrealtime ticks
+    uint32_t order[kPersistMaxR];  // one round's slots: kind << 12 | index
+};
+
+__device__ __forceinline__ uint32_t xcc_id() {
+    // HW_REG_XCC_ID (hardware register 20), bits [3:0]
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);
+}
+
+// An atomic read that is a read-modify-write (an add of a zero the compiler
+// cannot fold), so it is performed where the producers' atomic adds are.
+__device__ __forceinline__ uint32_t atomic_peek(uint32_t* p) {
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return __hip_atomic_fetch_add(p, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits (one thread) until *cnt >= need; false on the XCD's abort flag, the
+// time bound or the poll cap.
+__device__ __forceinline__ bool persist_wait(uint32_t* cnt, uint32_t need, uint32_t* abort_flag, uint64_t tmo) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (uint32_t poll = 0; poll < kPersistWaitCap; ++poll) {
+        if (atomic_peek(cnt) >= need) return true;
+        if (atomic_peek(abort_flag)) return false;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > tmo) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return false;
+}
+
+template <int LOGB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void persist30_kernel(const PersistArgs A) {
+    using S = InnerShape<LOGB>;
+    static_assert(S::WG == 256 && LOGB <= 12, "one 256-thread workgroup per item, blocks of at most 2^12");
+    using CL = CrtLds<4, kPersistCrtPitch, 28, 32, 2 * 8 * 8>;
+    union Lds {
+        DecompLds<4> d;
+        uint2 t[S::BPW * S::PADB];
+        CL c;
+    };
+    __shared__ Lds u;
+    __shared__ uint32_t s_next[2];
+    __shared__ uint32_t s_fail;
+    const uint32_t x = uniform(xcc_id() % A.nxcd);
+    uint32_t* xb = A.ctl + (size_t)x * A.xcd_words;
+    const uint32_t m_max = (A.batch + A.nxcd - 1) / A.nxcd;
+    uint32_t* dcnt = xb + kCtlCounts;
+    uint32_t* tcnt = xb + kCtlCounts + m_max;
+    const uint32_t m = persist_products_on(A.batch, A.nxcd, x);
+    const uint32_t slots = persist_slots(A.R, m, A.lag);
+    const uint32_t N = A.N, k = A.w.k, L = A.L;
+    const uint32_t total = A.batch << A.log_n;
+    const uint32_t q_inv = A.w.crt_mc->q_inv;
+    const uint32_t sub = threadIdx.x / S::TPB, tid = threadIdx.x % S::TPB;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t role = 3;  // the kind whose tables LDS holds (3: none)
+    uint32_t done = 0;
+    if (threadIdx.x == 0) {
+        s_next[0] = atomicAdd(&xb[kCtlHead], 1u);
+        s_fail = 0;
+    }
+    for (uint32_t iter = 0;; ++iter) {
+        __syncthreads();  // the previous item's LDS and s_next reads are done
+        const uint32_t it = uniform(s_next[iter & 1]);
+        if (it >= slots || s_fail) break;
+        uint32_t nxt = 0;  // the next queue position, fetched while this item runs
+        if (threadIdx.x == 0) nxt = atomicAdd(&xb[kCtlHead], 1u);
+        const PersistItem item = persist_decode(it, A.R, A.order, m, A.lag);
+        if (item.valid) {
+            const uint32_t kind = uniform(item.kind), idx = uniform(item.idx), lp = uniform(item.local);
+            const uint32_t P = x + A.nxcd * lp;  // the product
+            if (kind != kItemD) {
+                if (threadIdx.x == 0) {
+                    const bool ok = kind == kItemT ? persist_wait(&dcnt[lp], A.nD, &xb[kCtlAbort], A.tmo)
+                                                   : persist_wait(&tcnt[lp], A.nT, &xb[kCtlAbort], A.tmo);
+                    if (ok) {
+                        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+                    } else {
+                        atomicOr(&xb[kCtlErr], kPersistErrTimeout);
+                        atomicOr(A.sticky, kPersistErrTimeout);
+                        __hip_atomic_fetch_or(&xb[kCtlAbort], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        s_fail = 1;
+                    }
+                }
+                __syncthreads();
+                if (s_fail) break;
+            }
+            if (kind == kItemD) {
+                if (role != kItemD) {
+                    decompose30_mfma_tables<2>(u.d, A.w.mfma_tab, A.w.mfma_bias, A.w.pc, A.w.tw_fwd, k,
+                                               A.w.mfma_tiles, A.w.tw_log);
+                    __syncthreads();
+                    role = kItemD;
+                }
+                const uint32_t poly = idx / A.d_per_poly;  // 0: a, 1: b
+                const uint32_t chunk0 = (idx - poly * A.d_per_poly) * 4u;
+                const uint32_t* src = (poly ? A.b : A.a) + (size_t)P * N * L;
+                uint32_t* dst = (poly ? A.rb : A.ra) + (size_t)P * k * N;
+                decompose30_mfma_item<2, 1>(u.d, src, dst, chunk0, N, L, k, A.w.mfma_tiles);
+            } else if (kind == kItemT) {
+                role = kItemT;  // the exchanges overwrite any tables
+                uint32_t wid = idx * S::BPW + sub;  // (prime, block) of the product, 4 blocks per prime
+                if constexpr (S::TPB >= 64) wid = uniform(wid);
+                const bool active = wid < 4u * k;
+                if (!active) wid = 0;
+                const uint32_t j = wid >> 2, blk = wid & 3u;
+                const size_t off = ((size_t)P * k + j) * N + ((size_t)blk << LOGB);
+                const PrimeConst32 c = A.w.pc[j];
+                ntt_inner30_block<LOGB>(u.t + sub * S::PADB, tid, A.ra + off, A.rb + off, blk << LOGB, c,
+                                        A.w.tw_fwd + ((size_t)j << A.w.tw_log),
+                                        A.w.tw_inv + ((size_t)j << A.w.tw_log), A.log_n, active, false);
+            } else {
+                if (role != kItemC) {
+                    crt30_mfma_tables<4, 8, 28, 2, kPersistCrtPitch>(u.c, A.w.crt_tab, k, 8u, A.w.pc, A.w.tw_inv,
+                                                                    A.w.tw_log);
+                    __syncthreads();
+                    role = kItemC;
+                }
+                const uint32_t pos = idx * 256u + wave * 64u;  // the wave's first position in the product
+                if (pos < N)
+                    crt30_mfma_group<4, 8, 28, 2, kPersistCrtPitch, 32>(u.c, P * N + pos, A.ra, A.c, A.w.crt_c,
+                                                                       A.w.crt_mc, A.w.qc, A.log_n, L, k, 8u, total,
+                                                                       q_inv);
+            }
+            if (kind != kItemC) {  // publish: this item's stores are in L2, then count it
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_add(kind == kItemD ? &dcnt[lp] : &tcnt[lp], 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+            ++done;
+        }
+        if (threadIdx.x == 0) s_next[(iter + 1) & 1] = nxt;
+    }
+    if (threadIdx.x == 0) atomicAdd(&xb[kCtlDone], done);
+}
+
+__global__ __launch_bounds__(256) void persist_reset_kernel(uint32_t* __restrict__ ctl, uint32_t words) {
+    for (uint32_t i = threadIdx.x; i < words; i += 256) ctl[i] = 0u;
 }
 
 // ===========================================================================
@@ -1758,6 +2055,62 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     crt30_mfma_kernel<4, 8, 28, 2><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
                                                         d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, d.w30.pc,
                                                         d.w30.tw_inv, d.w30.tw_log);
+    return hipGetLastError();
+}
+
+bool persist_available(const DevPlan& d, uint32_t batch) {
+    return d.xcds >= 1 && d.xcds <= kPersistMaxXcd && batch >= d.xcds && d.log_n >= 10 && d.log_n <= 14 &&
+           crt_fuses_outer_inv(d, 30, true, 2) && decompose_fuses_outer(d, 30, true, 2) && d.w30.k <= 64 &&
+           persist_shape_for(d.log_n, d.w30.k).R <= kPersistMaxR;
+}
+
+size_t persist_ctl_bytes(const DevPlan& d, uint32_t batch) {
+    const uint32_t nxcd = d.xcds ? d.xcds : 1u;
+    return (size_t)nxcd * persist_xcd_words(batch, nxcd) * sizeof(uint32_t);
+}
+
+hipError_t launch_persist30(const DevPlan& d, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t* ra,
+                            uint32_t* rb, uint32_t batch, uint32_t* ctl, uint32_t* sticky, uint64_t tmo,
+                            hipStream_t s) {
+    if (!persist_available(d, batch)) return hipErrorInvalidValue;
+    const PersistShape sh = persist_shape_for(d.log_n, d.w30.k);
+    PersistArgs A{};
+    A.a = a;
+    A.b = b;
+    A.c = c;
+    A.ra = ra;
+    A.rb = rb;
+    A.ctl = ctl;
+    A.sticky = sticky;
+    A.w = d.w30;
+    A.N = d.N;
+    A.log_n = d.log_n;
+    A.L = d.L;
+    A.batch = batch;
+    A.nxcd = d.xcds;
+    A.xcd_words = persist_xcd_words(batch, d.xcds);
+    A.R = sh.R;
+    A.nD = sh.nD;
+    A.nT = sh.nT;
+    A.nC = sh.nC;
+    A.d_per_poly = sh.nD / 2;
+    A.tmo = tmo;
+    persist_order(sh, A.order);
+    const uint32_t words = d.xcds * A.xcd_words;
+    persist_reset_kernel<<<1, 256, 0, s>>>(ctl, words);
+    // three workgroups per compute unit (53 KB of LDS and 168 VGPRs each);
+    // workgroups go round-robin over the XCDs, so each XCD's queue gets
+    // 3 cus / xcds of them
+    const dim3 grid(3 * d.cus);
+    A.lag = persist_lag(sh.R, grid.x / d.xcds);
+    switch (d.log_n) {
+        case 10: persist30_kernel<8><<<grid, 256, 0, s>>>(A); break;
+        case 11: persist30_kernel<9><<<grid, 256, 0, s>>>(A); break;
+        case 12: persist30_kernel<10><<<grid, 256, 0, s>>>(A); break;
+        case 13: persist30_kernel<11><<<grid, 256, 0, s>>>(A); break;
+        case 14: persist30_kernel<12><<<grid, 256, 0, s>>>(A); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "nc_common.hpp"
+#include "nc_persist.hpp"
 
 namespace nc {
 
@@ -50,6 +51,7 @@ struct DevSet30 {
 struct DevPlan {
     uint32_t N = 0, L = 0, log_n = 0;
     uint32_t cus = 256;  // compute units of the device
+    uint32_t xcds = 0;   // its XCDs (0: unknown)
     DevSet62 w62;
     DevSet30 w30;
 };
@@ -105,6 +107,22 @@ hipError_t launch_twiddles30(Tw32* fwd, Tw32* inv, const Tw32* pow, const PrimeC
 // shape L = 28 with 57..64 primes.
 bool crt_fuses_outer_inv(const DevPlan& d, int width, bool mfma, uint32_t log_blocks);
 hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out, uint32_t nprod, hipStream_t s);
+
+// The persistent batched form (nc_persist.hpp): one launch per batch of
+// `batch` products at this plan's N.  Available for the 30-bit set with both
+// matrix-core paths at outer split 2, L = 28, 57..64 primes, N = 2^10..2^14,
+// a known XCD count (at most kPersistMaxXcd) and batch >= that count.
+bool persist_available(const DevPlan& d, uint32_t batch);
+// Control block bytes of one launch (placed by the caller in device memory
+// that no other launch uses meanwhile).
+size_t persist_ctl_bytes(const DevPlan& d, uint32_t batch);
+// Clears the control block and launches the kernel: ra / rb hold `batch`
+// products' residue rows ([batch][k][N] u32 each), sticky is the plan's
+// error word, tmo the bound of one readiness wait in s_memrealtime ticks
+// (100 MHz).
+hipError_t launch_persist30(const DevPlan& d, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t* ra,
+                            uint32_t* rb, uint32_t batch, uint32_t* ctl, uint32_t* sticky, uint64_t tmo,
+                            hipStream_t s);
 
 // Table-set initialisation on the device.
 // Candidates top - i step, i < count (top = 1 mod step): out[i] = 0 if not

@@ -108,6 +108,7 @@ struct Choice {
     uint32_t log_blocks = 0;    // NTT outer split
     uint32_t mfma = kMfmaAll;   // int8 matrix-core paths where available
     uint32_t chunks = 1;        // batch split into this many chunks on two staggered streams
+    uint32_t persist = 0;       // 1: the persistent batched form (one launch per batch) where available
 };
 
 }  // namespace nc
@@ -160,5 +161,11 @@ struct nc_plan {
     bool forced = false;
     nc::Choice forced_choice;
     int mfma_mode = -1;  // -1: tuned (default all on), else a kMfma* mask (nc_plan_set_mfma)
+    // persistent batched form: the error word its launches set when a
+    // readiness wait reaches its bound (read and cleared by
+    // nc_plan_persist_status), the pinned mode and the wait bound
+    uint32_t* d_err = nullptr;
+    int persist_mode = -1;                // -1: tuned (default off), 0: never, 1: wherever available
+    uint64_t persist_tmo = 100000000ull;  // s_memrealtime ticks (100 MHz): 1 s
     uint32_t chunks_forced = 0;  // 0: tuned (default 1), else every batch uses this many chunks
 };

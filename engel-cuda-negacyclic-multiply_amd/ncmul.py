@@ -103,6 +103,12 @@ def lib():
         L.nc_plan_set_mfma.argtypes = [_vp, ctypes.c_int]
         L.nc_plan_get_choice.argtypes = [_vp, ctypes.c_uint32] + [ctypes.POINTER(ctypes.c_uint32)] * 4
         L.nc_plan_set_chunks.argtypes = [_vp, ctypes.c_uint32]
+        L.nc_plan_set_persist.argtypes = [_vp, ctypes.c_int, ctypes.c_uint32]
+        L.nc_plan_get_persist.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.nc_plan_persist_status.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.nc_persist_emulate.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
         L.nc_multiply.argtypes = [_vp, _u32p, _u32p, _u32p, ctypes.c_size_t]
         L.nc_multiply_device.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
         L.nc_multiply_device_noalloc.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint32, _vp]
@@ -187,6 +193,14 @@ def host_twiddles(N: int, q: np.ndarray, width: int, inverse: bool) -> np.ndarra
 def set_tune_cache(path):
     """Process-wide autotuner cache file (None disables; NC_TUNE_CACHE otherwise)."""
     _check(lib().nc_set_tune_cache(None if path is None else str(path).encode()), "nc_set_tune_cache")
+
+
+def persist_emulate(N: int, k: int, batch: int, xcds: int, workgroups: int, seed: int = 1) -> dict:
+    """CPU emulation of the persistent form's queues (raises NcError on a failed check)."""
+    mk, wf = ctypes.c_double(), ctypes.c_double()
+    _check(lib().nc_persist_emulate(N, k, batch, xcds, workgroups, seed, ctypes.byref(mk), ctypes.byref(wf)),
+           "nc_persist_emulate")
+    return {"makespan": float(mk.value), "wait_fraction": float(wf.value)}
 
 
 def measure_mul32_peak() -> float:
@@ -310,6 +324,23 @@ class Plan:
     def set_chunks(self, chunks: int):
         """Batch slices on two staggered streams: 0 tuned, else forced."""
         _check(lib().nc_plan_set_chunks(self._h, int(chunks)), "nc_plan_set_chunks")
+
+    def set_persist(self, mode: int, wait_bound_us: int = 0):
+        """Persistent batched form: -1 tuned, 0 never, 1 wherever available; bound of one in-launch wait."""
+        _check(lib().nc_plan_set_persist(self._h, int(mode), int(wait_bound_us)), "nc_plan_set_persist")
+
+    def persistent(self, batch: int) -> bool:
+        """Whether calls with `batch` products run the persistent form."""
+        v = ctypes.c_uint32()
+        _check(lib().nc_plan_get_persist(self._h, batch, ctypes.byref(v)), "nc_plan_get_persist")
+        return bool(v.value)
+
+    def persist_status(self, stream=None) -> int:
+        """Error bits of the persistent launches since the last call (0: all drained); clears them."""
+        v = ctypes.c_uint32()
+        _check(lib().nc_plan_persist_status(self._h, _stream_handle(stream), ctypes.byref(v)),
+               "nc_plan_persist_status")
+        return int(v.value)
 
     def set_mfma(self, mode: int):
         """Matrix-core paths: a mask of MFMA_DECOMPOSE | MFMA_CRT (0: VALU only), or -1 tuned."""

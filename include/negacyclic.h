@@ -164,6 +164,42 @@ int nc_plan_set_chunks(nc_plan* plan, uint32_t chunks);
 #define NC_MFMA_CRT 2
 int nc_plan_set_mfma(nc_plan* plan, int mode);
 
+/* Persistent batched form: the whole batch in one launch, replacing the
+ * reference's fixed launch sequence per product (solve.cu:1489-1528).  Every
+ * XCD drains its own queue of work items -- decompositions (with the first
+ * two forward NTT stages), inner NTT blocks with the pointwise product, CRT
+ * groups (with the last two inverse stages) -- of the products assigned to
+ * it, so the stages of different products share every compute unit; an NTT
+ * item waits for its product's decompositions, a CRT item for its product's
+ * NTT items, through counters polled inside the launch.  Available for the
+ * 30-bit set with both matrix-core paths (the fused shape: L = 28, 57..64
+ * primes), N = 2^10 .. 2^14, when the device reports its XCD count and the
+ * batch has at least one product per XCD; elsewhere calls run the launch
+ * sequence.  mode: -1 lets nc_plan_autotune decide (default off until
+ * tuned), 0 never, 1 wherever available.  wait_bound_us bounds one readiness
+ * wait inside the launch (0: 1 s); a wait that reaches it stops its XCD's
+ * queue, so the launch still ends, and sets the plan's error word
+ * (nc_plan_persist_status).  Output bits never depend on the mode.
+ * (No counterpart in the reference.) */
+int nc_plan_set_persist(nc_plan* plan, int mode, uint32_t wait_bound_us);
+/* *persistent = 1 if calls with `batch` products run the persistent form. */
+int nc_plan_get_persist(const nc_plan* plan, uint32_t batch, uint32_t* persistent);
+/* Synchronises `stream` and the plan's last workspace user, then returns in
+ * *flags and clears the error bits the plan's persistent launches set since
+ * the last call (1: a readiness wait reached its bound; the products of that
+ * launch are not valid).  0 when every launch drained its queues. */
+int nc_plan_persist_status(nc_plan* plan, void* stream, uint32_t* flags);
+/* CPU emulation of the persistent form's queues (no device): `batch`
+ * products of degree N with k primes over `xcds` XCDs, `workgroups`
+ * workgroups per XCD taking queue positions in order (each fetching its next
+ * position before running the current one) with random item durations from
+ * `seed`.  NC_OK when every item ran exactly once, only after its inputs,
+ * and every queue drained (no deadlock); NC_ERR_STATE naming the failed
+ * check otherwise.  makespan (emulated time units) and wait_fraction (share
+ * of workgroup time spent waiting for inputs) may be NULL. */
+int nc_persist_emulate(uint32_t N, uint32_t k, uint32_t batch, uint32_t xcds, uint32_t workgroups, uint64_t seed,
+                       double* makespan, double* wait_fraction);
+
 /* Autotuner: times every (prime width, NTT split, matrix-core paths, chunks)
  * for `batch` products on `stream` (zero inputs; the kernels' work does not
  * depend on the data), twice each, then the three fastest and the fastest

@@ -1,0 +1,153 @@
+"""GPU parity of the persistent batched form (nc_plan_set_persist): one
+launch per batch, every XCD draining a queue of decomposition, NTT and CRT
+items, against the oracle's restatement of the reference (solve.cu) and the
+plain launch sequence, bit for bit, through the C ABI.
+
+Sizes: every degree the form covers (N = 2^10 .. 2^14) at batch 8 (one
+product per XCD), uneven batches, and the 128-product headline batch at
+N = 2^14 with every product checked; edge inputs (all q - 1, zeros); the
+error path forced from the host with a 1 us readiness bound.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ncmul
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+L = 28
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def batch_inputs(N, B, q, seed):
+    a = np.stack([ncmul.gen_uniform(seed + 2 * i, N, q) for i in range(B)])
+    b = np.stack([ncmul.gen_uniform(seed + 2 * i + 1, N, q) for i in range(B)])
+    return a, b
+
+
+def run_device(plan, a, b):
+    B, N = a.shape[0], a.shape[1]
+    d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+    d_c = ncmul.empty_device_u32((B, N, L))
+    d_c.fill_(-1)  # products the launch leaves out would show as all ones
+    plan.multiply_device(d_a, d_b, d_c, B)
+    return ncmul.from_device_u32(d_c)
+
+
+@pytest.mark.parametrize("N,B", [(1024, 8), (1024, 19), (2048, 8), (4096, 11), (8192, 8), (8192, 24),
+                                 (16384, 8), (16384, 13)])
+def test_persist_matches_oracle(gpu, N, B, q868):
+    say("persist", N, B)
+    a, b = batch_inputs(N, B, q868, 7000 + N + B)
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_persist(1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+        for i in range(B):
+            assert np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0)), i
+        # a smaller batch on the same workspace: the control block starts clean
+        B2 = max(8, B // 2)
+        got2 = run_device(plan, b[:B2], a[:B2])
+        assert plan.persist_status() == 0
+        for i in range(B2):
+            assert np.array_equal(got2[i], got[i]), i
+
+
+def test_persist_headline_batch(gpu, q868):
+    """128 products at N = 2^14 (the headline shape), every one against the
+    oracle, then twice more on the same buffers (launch-to-launch reuse)."""
+    N, B = 16384, 128
+    say("persist headline", N, B)
+    a, b = batch_inputs(N, B, q868, 9100)
+    orc = O.Plan(N, q868)
+    want = [orc.multiply(a[i], b[i], threads=0) for i in range(B)]
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_persist(1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+        d_c = ncmul.empty_device_u32((B, N, L))
+        for rep in range(3):
+            d_c.fill_(-1)
+            plan.multiply_device(d_a, d_b, d_c, B)
+            got = ncmul.from_device_u32(d_c)
+            assert plan.persist_status() == 0, rep
+            bad = [i for i in range(B) if not np.array_equal(got[i], want[i])]
+            assert not bad, (rep, bad[:8])
+
+
+def test_persist_equals_launch_sequence(gpu, q868):
+    """The persistent form and the plain launch sequence give the same bits,
+    on random inputs and on the reference's edge inputs (all coefficients
+    q - 1, all zero)."""
+    N, B = 16384, 16
+    say("persist vs sequence", N, B)
+    a, b = batch_inputs(N, B, q868, 9300)
+    qm1 = np.tile(q868 - np.array([1] + [0] * (L - 1), dtype=np.uint32), (N, 1))
+    a[0], b[0] = qm1, qm1
+    a[1] = 0
+    b[2] = qm1
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_persist(0)
+        plan.set_tuning(30, 2)
+        want = run_device(plan, a, b)
+        plan.set_persist(1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+    assert np.array_equal(got, want)
+    # closed forms: (q-1)^2 summed negacyclically gives c_k = 2k + 2 - N mod q
+    qi = O.limbs_to_int(q868)
+    for k_ in (0, 1, N // 2, N - 1):
+        assert O.limbs_to_int(got[0][k_]) == (2 * k_ + 2 - N) % qi
+    assert not got[1].any()
+
+
+def test_persist_wait_bound_ends_launch(gpu, q868):
+    """A readiness bound of 1 us, forced from the host, makes the NTT items
+    of a one-product-per-XCD batch give up while the decompositions run: the
+    launch still ends, the plan's error word says so, and the next launch
+    with the normal bound is exact."""
+    N, B = 16384, 8
+    say("persist wait bound", N, B)
+    a, b = batch_inputs(N, B, q868, 9500)
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_persist(1, wait_bound_us=1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        run_device(plan, a, b)
+        assert plan.persist_status() & 1
+        assert plan.persist_status() == 0  # read and cleared
+        plan.set_persist(1)
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+        for i in (0, B - 1):
+            assert np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0)), i
+
+
+def test_persist_autotune_candidate(gpu, q868):
+    """The autotuner times the persistent form at a headline-sized batch and
+    keeps it only if its runs drained every queue; whatever it picks, the
+    output stays exact."""
+    N, B = 16384, 64
+    say("persist autotune", N, B)
+    a, b = batch_inputs(N, B, q868, 9700)
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.autotune(B)
+        say("persistent chosen:", plan.persistent(B))
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+        for i in (0, 17, B - 1):
+            assert np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0)), i
