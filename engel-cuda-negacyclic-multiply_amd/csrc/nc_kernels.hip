@@ -184,6 +184,25 @@ __device__ __forceinline__ void ct_pass(typename A::T (&v)[NE], uint32_t gx0, in
 // of a group taking prime tiles [w T / WS, (w + 1) T / WS) of the T tiles,
 // so each wave's serial chain is WS times shorter; a work item then holds
 // 4 / WS chunks.
+// Copies n 16-byte pieces from global memory to LDS (a 256-thread
+// workgroup, at most MAXPER pieces per thread): every load is issued before
+// the first store, and no load sits in a branch (a thread past the end
+// re-reads piece n - 1), so the copy costs about one memory round trip.
+template <int MAXPER, class Src, class Dst>
+__device__ __forceinline__ void copy16_to_lds(uint32_t n, Src src_of, Dst dst_of) {
+    nc_v4i t[MAXPER];
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i) {
+        const uint32_t w = threadIdx.x + 256u * i;
+        t[i] = *src_of(w < n ? w : n - 1);
+    }
+#pragma unroll
+    for (int i = 0; i < MAXPER; ++i) {
+        const uint32_t w = threadIdx.x + 256u * i;
+        if (w < n) *dst_of(w) = t[i];
+    }
+}
+
 struct DecompPrimeLds {
     uint32_t p, mont_neg;
 };
@@ -208,11 +227,13 @@ __device__ __forceinline__ void decompose30_mfma_tables(DecompLds<1 << S1>& s, c
                                                         const Tw32* __restrict__ twf, uint32_t k, uint32_t tiles,
                                                         uint32_t tw_log) {
     constexpr int G = 1 << S1;
-    for (uint32_t w = threadIdx.x; w < tiles * 32 * (kMfmaK / 16); w += 256) {
-        const uint32_t row = w / (kMfmaK / 16), chunk = w % (kMfmaK / 16);
-        *reinterpret_cast<nc_v4i*>(s.tab + row * kMfmaRowStride + chunk * 16) =
-            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kMfmaK + chunk * 16);
-    }
+    constexpr int kPieces = kMfmaTiles * 32 * (kMfmaK / 16);
+    copy16_to_lds<(kPieces + 255) / 256>(
+        tiles * 32 * (kMfmaK / 16),
+        [&](uint32_t w) { return reinterpret_cast<const nc_v4i*>(tab + (size_t)w * 16); },
+        [&](uint32_t w) {
+            return reinterpret_cast<nc_v4i*>(s.tab + (w / (kMfmaK / 16)) * kMfmaRowStride + (w % (kMfmaK / 16)) * 16);
+        });
     for (uint32_t r = threadIdx.x; r < tiles * 32; r += 256) s.bias[r] = bias[r];
     for (uint32_t j = threadIdx.x; j < tiles * 8; j += 256) {
         s.prime[j] = j < k ? DecompPrimeLds{pc[j].p, pc[j].mont_neg} : DecompPrimeLds{1u, 0u};
@@ -1242,11 +1263,13 @@ __device__ __forceinline__ void crt30_mfma_tables(Lds& s, const int8_t* __restri
     if (KCT > 0) kc_n = KCT;
     const uint32_t kbytes = 32 * kc_n;
     const uint32_t chunks = kbytes / 16;
-    for (uint32_t w = threadIdx.x; w < RT * 32 * chunks; w += 256) {
-        const uint32_t row = w / chunks, ch = w - row * chunks;
-        *reinterpret_cast<nc_v4i*>(s.tab + row * PITCH + ch * 16) =
-            *reinterpret_cast<const nc_v4i*>(tab + (size_t)row * kbytes + ch * 16);
-    }
+    // the table is row-major with rows of kbytes: piece w is at byte 16 w
+    copy16_to_lds<(RT * 32 * KCM * 2 + 255) / 256>(
+        RT * 32 * chunks, [&](uint32_t w) { return reinterpret_cast<const nc_v4i*>(tab + (size_t)w * 16); },
+        [&](uint32_t w) {
+            const uint32_t row = w / chunks, ch = w - row * chunks;
+            return reinterpret_cast<nc_v4i*>(s.tab + row * PITCH + ch * 16);
+        });
     // entries 8 kc + 4 h + e for every chunk; rows >= k repeat row k - 1
     if constexpr (INV > 0) {
         for (uint32_t r2 = threadIdx.x; r2 < 16u * KCM; r2 += 256) {
@@ -1553,8 +1576,12 @@ __global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
 // stores are complete in that L2 once its waves pass s_waitcnt vmcnt(0); the
 // workgroup then bumps its product's counter with an atomic add.  A consumer
 // polls the counter with atomic read-modify-writes (which meet the adds at the
-// same place), then invalidates its compute unit's L1 (buffer_inv sc0) before
-// it loads.  Each wait is bounded in time (s_memrealtime, 100 MHz) and by an
+// same place) before it loads.  A T item's compute unit has not read its
+// input lines before in this launch (L1 starts empty at every launch) and
+// the D items write them as whole 128-byte lines, so any copy its own stores
+// left in L1 is current.  A C item's rows were rewritten in place by T items,
+// perhaps on other compute units after this one's D item wrote them, so it
+// first invalidates its L1 (buffer_inv sc0).  Each wait is bounded in time (s_memrealtime, 100 MHz) and by an
 // iteration cap: on expiry the workgroup sets the XCD's abort flag and error
 // bits and the plan's error word, and every workgroup of that XCD stops, so
 // the launch always ends.
@@ -1650,13 +1677,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
                 if (threadIdx.x == 0) {
                     const bool ok = kind == kItemT ? persist_wait(&dcnt[lp], A.nD, &xb[kCtlAbort], A.tmo)
                                                    : persist_wait(&tcnt[lp], A.nT, &xb[kCtlAbort], A.tmo);
-                    if (ok) {
-                        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
-                    } else {
+                    if (!ok) {
                         atomicOr(&xb[kCtlErr], kPersistErrTimeout);
                         atomicOr(A.sticky, kPersistErrTimeout);
                         __hip_atomic_fetch_or(&xb[kCtlAbort], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         s_fail = 1;
+                    } else if (kind == kItemC) {
+                        // a C item's rows were rewritten in place by T items
+                        // on other compute units: drop this one's L1 copies
+                        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
                     }
                 }
                 __syncthreads();

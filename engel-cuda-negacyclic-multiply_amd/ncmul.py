@@ -509,6 +509,23 @@ class Batch:
         g = _groups(groups)
         _check(lib().nc_batch_run(self._h, g, len(groups), _stream_handle(stream)), "nc_batch_run")
 
+    def set_persist(self, degrees, mode: int, wait_bound_us: int = 0):
+        """nc_plan_set_persist on the batch's plan of each degree."""
+        for N in degrees:
+            h = _vp()
+            _check(lib().nc_batch_plan(self._h, int(N), ctypes.byref(h)), "nc_batch_plan")
+            _check(lib().nc_plan_set_persist(h, int(mode), int(wait_bound_us)), "nc_plan_set_persist")
+
+    def persist_status(self, degrees) -> int:
+        """OR of nc_plan_persist_status over the batch's plans of these degrees."""
+        flags = 0
+        for N in degrees:
+            h, v = _vp(), ctypes.c_uint32()
+            _check(lib().nc_batch_plan(self._h, int(N), ctypes.byref(h)), "nc_batch_plan")
+            _check(lib().nc_plan_persist_status(h, None, ctypes.byref(v)), "nc_plan_persist_status")
+            flags |= int(v.value)
+        return flags
+
     def capture(self, groups) -> Graph:
         g = _groups(groups)
         h = ctypes.c_void_p()

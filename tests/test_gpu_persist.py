@@ -151,3 +151,32 @@ def test_persist_autotune_candidate(gpu, q868):
         assert plan.persist_status() == 0
         for i in (0, 17, B - 1):
             assert np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0)), i
+
+
+def test_persist_in_mixed_degree_graph(gpu, q868):
+    """The mixed-degree graph batch with the persistent form on for every
+    degree: one persistent launch per group of 2^10 .. 2^14 (groups below
+    one product per XCD and 2^15 keep the launch sequence), captured in one
+    HIP graph, three replays with eager work between them, every product
+    equal to the plain path's."""
+    import torch
+    from test_gpu_batch_serve import make_groups, mixed_degrees, plain_products
+
+    degrees = mixed_degrees(256, 10, 15, seed=14)
+    say("persist graph degrees", sorted(set(degrees)))
+    groups, host = make_groups(degrees, q868, 4000)
+    want = plain_products(q868, host)
+    all_n = sorted(set(degrees))
+    with ncmul.Batch(q868) as batch:
+        batch.set_persist(all_n, 1)
+        graph = batch.capture(groups)
+        for rep in range(3):
+            for _, _, _, _, c in groups:
+                c.fill_(-1)
+            graph.launch()
+            torch.cuda.synchronize()
+            assert batch.persist_status(all_n) == 0, rep
+            for N, n, a, b, c in groups:
+                assert np.array_equal(ncmul.from_device_u32(c), want[N]), (rep, N)
+            batch.run(groups[:2])
+            torch.cuda.synchronize()
