@@ -145,6 +145,22 @@ def mixed_batch(q: np.ndarray, n: int, seed: int, launches: int):
     torch.cuda.synchronize()
     graph_s = (time.perf_counter() - t0) / launches
     got = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
+    # the same graph with the persistent form on for every degree it covers
+    # (2^10 .. 2^14 groups of at least one product per XCD)
+    degs = sorted({N for N, _, _, _, _ in groups})
+    batch.set_persist(degs, 1)
+    pgraph = batch.capture(groups)
+    for _ in range(10):
+        pgraph.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        pgraph.launch()
+    torch.cuda.synchronize()
+    pgraph_s = (time.perf_counter() - t0) / launches
+    pgot = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
+    p_errors = batch.persist_status(degs)
+    pgraph.close()
     # plain: one product per call on resident operands, per-degree plans tuned for one product
     plains = {}
     for N, cnt, d_a, d_b, _ in groups:
@@ -171,7 +187,10 @@ def mixed_batch(q: np.ndarray, n: int, seed: int, launches: int):
         "degrees": {str(N): cnt for N, cnt, _, _, _ in groups},
         "products_per_s": n / graph_s, "ms_per_launch": graph_s * 1e3,
         "plain_products_per_s": n / plain_s, "over_plain": plain_s / graph_s,
-        "bit_exact_vs_plain": bool(exact),
+        "bit_exact_vs_plain": bool(exact and p_errors == 0 and all(np.array_equal(pgot[N], got[N]) for N in got)),
+        "persistent_graph": {"products_per_s": n / pgraph_s, "ms_per_launch": pgraph_s * 1e3,
+                             "wait_errors": p_errors,
+                             "same_output": bool(all(np.array_equal(pgot[N], got[N]) for N in got))},
     }
 
 
@@ -552,6 +571,28 @@ def main() -> int:
         "same_output_as_30_bit": bool(np.array_equal(c62, c_batch)),
     }
 
+    # ---- the persistent batched form: the whole step in one launch whose
+    # workgroups drain per-XCD queues of decomposition, NTT and CRT items ----
+    pp = ncmul.Plan(N, q)
+    pp.set_persist(1)
+    persistent = {"available": pp.persistent(B), "chosen_by_autotuner": batched.persistent(B)}
+    if persistent["available"]:
+        for _ in range(5):
+            pp.multiply_device(d_a, d_b, d_c, B)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            pp.multiply_device(d_a, d_b, d_c, B)
+        torch.cuda.synchronize()
+        ms_p = (time.perf_counter() - t0) / args.steps * 1e3
+        persistent.update({
+            "ms_per_step": ms_p, "products_per_s": B / (ms_p * 1e-3),
+            "vs_launch_sequence": (elapsed / args.steps * 1e3) / ms_p,
+            "wait_errors": pp.persist_status(stream),
+            "same_output_as_launch_sequence": bool(np.array_equal(ncmul.from_device_u32(d_c), c_batch)),
+        })
+    pp.close()
+
     # bit-exactness of every product of the batched step against the CPU
     # restatement of the reference (the checker, after the timed region)
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -667,6 +708,7 @@ def main() -> int:
         "roofline_hbm": roofline_hbm,
         "stage_roofline": stage_roofline,
         "width62": width62,
+        "persistent": persistent,
         "w_convention": {
             "W_per_product": W,
             "whole_path_mul32_per_s": batched_pps * W,
@@ -681,7 +723,8 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, q, args.cpu_seconds)
     print(json.dumps(out), flush=True)
-    ok = bit_exact and width62["same_output_as_30_bit"] and all(
+    ok = bit_exact and width62["same_output_as_30_bit"] and persistent.get("same_output_as_launch_sequence", True) \
+        and not persistent.get("wait_errors", 0) and all(
         out[k]["bit_exact_vs_plain"] for k in ("mixed_batch", "served") if k in out)
     return 0 if ok else 3
 
