@@ -1270,7 +1270,8 @@ int nc_multiply_device_noalloc(nc_plan* p, const uint32_t* d_a, const uint32_t* 
                                          ") exceeds the scratch array (" + std::to_string(nc::kScratchBytes) + " bytes)");
     // residue rows of a in the first half of what the batch uses, b after
     // them: nothing is allocated, and nothing needs to start at zero (every
-    // residue is written before it is read)
+    // residue is written before it is read; the persistent form's control
+    // block, in b's trailer, is cleared by that form's reset kernel)
     char* ws = static_cast<char*>(p->scratch);
     Choice c = choose(p, batch);
     c.chunks = 1;

@@ -253,7 +253,9 @@ int nc_multiply_device(nc_plan* plan, const uint32_t* d_a, const uint32_t* d_b, 
  * library routine.  batch * nc_plan_info.bytes_per_product must not exceed
  * NC_NOALLOC_SCRATCH_BYTES (NC_ERR_CAPACITY otherwise): one product at
  * N = 65536, four at N = 16384.  Every call on one device shares that
- * device's array and nothing in it needs to start at zero; calls that
+ * device's array and nothing in it needs to start at zero (the persistent
+ * form's control block, in the array's per-product trailer, is cleared by
+ * that form's own first kernel, not by a memset); calls that
  * overlap on one device (other streams, plans or threads) race on it, so
  * the caller orders them. */
 #define NC_NOALLOC_SCRATCH_BYTES (32ull << 20)

@@ -180,3 +180,28 @@ def test_persist_in_mixed_degree_graph(gpu, q868):
                 assert np.array_equal(ncmul.from_device_u32(c), want[N]), (rep, N)
             batch.run(groups[:2])
             torch.cuda.synchronize()
+
+
+def test_persist_noalloc_scratch(gpu, q868):
+    """The allocation-free entry point with the persistent form on: its
+    control block sits in the scratch array's per-product trailer and is
+    cleared by the form's reset kernel; two calls in a row, the second
+    smaller, each equal to the plain launch sequence through
+    nc_multiply_device."""
+    N = 1024
+    say("persist noalloc", N)
+    with ncmul.Plan(N, q868) as plan:
+        for B in (16, 9):
+            a, b = batch_inputs(N, B, q868, 9900 + B)
+            plan.set_persist(0)
+            want = run_device(plan, a, b)
+            plan.set_persist(1)
+            if not plan.persistent(B):
+                pytest.skip("the device does not report its XCD count")
+            d_a, d_b = ncmul.to_device_u32(a), ncmul.to_device_u32(b)
+            d_c = ncmul.empty_device_u32((B, N, L))
+            d_c.fill_(-1)
+            plan.multiply_device_noalloc(d_a, d_b, d_c, B)
+            got = ncmul.from_device_u32(d_c)
+            assert plan.persist_status() == 0, B
+            assert np.array_equal(got, want), B
