@@ -698,6 +698,9 @@ def main() -> int:
             "p50_ms": p50, "p99_ms": p99, "calls": calls,
         },
         "batched_over_plain": batched_pps / plain_dev_pps,
+        # the same rate as time per 128 products (the batch the kernel figures
+        # at batch 128 are quoted at)
+        "ms_per_128_products": 128e3 / batched_pps,
         "plan_create_ms": create_ms,
         "tuning": tuning,
         "p99_ms": p99,
