@@ -1581,10 +1581,10 @@ __global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
 // the D items write them as whole 128-byte lines, so any copy its own stores
 // left in L1 is current.  A C item's rows were rewritten in place by T items,
 // perhaps on other compute units after this one's D item wrote them, so it
-// first invalidates its L1 (buffer_inv sc0).  Each wait is bounded in time (s_memrealtime, 100 MHz) and by an
-// iteration cap: on expiry the workgroup sets the XCD's abort flag and error
-// bits and the plan's error word, and every workgroup of that XCD stops, so
-// the launch always ends.
+// first invalidates its L1 (buffer_inv sc0).  Each wait is bounded in time
+// (s_memrealtime, 100 MHz) and by an iteration cap: on expiry the workgroup
+// sets the XCD's abort flag and error bits and the plan's error word, and
+// every workgroup of that XCD stops, so the launch always ends.
 // ===========================================================================
 constexpr int kPersistCrtPitch = 8 * 32 + 16;  // CRT table pitch for 8 chunks of 8 primes (16-byte padded)
 constexpr uint32_t kPersistWaitCap = 1u << 22;  // polls per wait at most (the time bound comes first)
