@@ -4,9 +4,10 @@ items, against the oracle's restatement of the reference (solve.cu) and the
 plain launch sequence, bit for bit, through the C ABI.
 
 Sizes: every degree the form covers (N = 2^10 .. 2^14) at batch 8 (one
-product per XCD), uneven batches, and the 128-product headline batch at
-N = 2^14 with every product checked; edge inputs (all q - 1, zeros); the
-error path forced from the host with a 1 us readiness bound.
+product per XCD), uneven batches, the 128- and 256-product batches at
+N = 2^14 with every product checked, a second 868-bit modulus; edge inputs
+(all q - 1, zeros); the mixed-degree graph and the allocation-free entry
+point; the error path forced from the host with a 1 us readiness bound.
 """
 from __future__ import annotations
 
@@ -205,3 +206,41 @@ def test_persist_noalloc_scratch(gpu, q868):
             got = ncmul.from_device_u32(d_c)
             assert plan.persist_status() == 0, B
             assert np.array_equal(got, want), B
+
+
+def test_persist_random_prime_modulus(gpu):
+    """A second 868-bit modulus (a seeded random prime, whose reduction the
+    reference's own Barrett step gets wrong): every product of an uneven
+    batch against the oracle's exact reduction."""
+    from conftest import modulus
+
+    q = modulus("q868b")
+    N, B = 16384, 11
+    say("persist q868b", N, B)
+    a, b = batch_inputs(N, B, q, 9800)
+    orc = O.Plan(N, q)
+    with ncmul.Plan(N, q) as plan:
+        plan.set_persist(1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+        for i in range(B):
+            assert np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0)), i
+
+
+def test_persist_batch_256(gpu, q868):
+    """The bench's headline batch, 256 products at N = 2^14 (32 per XCD),
+    every product against the oracle."""
+    N, B = 16384, 256
+    say("persist batch 256", N, B)
+    a, b = batch_inputs(N, B, q868, 12000)
+    orc = O.Plan(N, q868)
+    with ncmul.Plan(N, q868) as plan:
+        plan.set_persist(1)
+        if not plan.persistent(B):
+            pytest.skip("the device does not report its XCD count")
+        got = run_device(plan, a, b)
+        assert plan.persist_status() == 0
+    bad = [i for i in range(B) if not np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0))]
+    assert not bad, bad[:8]
