@@ -2502,7 +2502,7 @@ hipError_t launch_host_xfer(const void* const* src, void* const* dst, uint32_t c
 // process; host code uses the address hipGetSymbolAddress returns for the
 // current device, never the array's name.
 // ===========================================================================
-__device__ __align__(256) unsigned char scratch[kScratchBytes];
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 hipError_t scratch_address(void** out) {
     constexpr int kMaxDevices = 64;
