@@ -89,7 +89,7 @@ constexpr int kCrtMaxKC = (kMaxPrimes30 + 7) / 8;   // contraction chunks (8 pri
 constexpr int kCrtRowStride = kCrtMaxKC * 32 + 16;  // LDS row pitch: conflict-free 16-byte reads
 constexpr int kCrtMaxRT = 5;                        // (32 + 4) pseudo-limbs / 8 per row tile
 
-// Montgomery finish of the MFMA CRT (R = 2^96): the table carries
+// Montgomery finish of the MFMA CRT (R = 2^32): the table carries
 // (M / p_j) mod q times R, so the limb sum x satisfies x = c R (mod q).
 struct CrtMfmaConst {
     uint32_t neg_m_r[kMaxLimbs];  // ((-M) mod q) R mod q

@@ -1172,21 +1172,21 @@ __global__ __launch_bounds__(256) void crt30_kernel(const uint32_t* __restrict__
     wave_store_limbs<LT>(acc, lds_out + (threadIdx.x - lane) * LT, out + (size_t)wbase * L, lane, L, count);
 }
 
-// x -> x 2^-96 mod q for x < q 2^96 (NA limbs), three word-level Montgomery
-// steps and one conditional subtraction, in place: the LQ limbs of the
-// result (zero beyond L) end in x[3 .. 3 + LQ).  The subtraction runs as a
-// borrow-only pass (r >= q ?) and a masked pass, so no second limb array
+// x -> x 2^-32 mod q for x < (2^32 - 1) q (NA limbs), one word-level
+// Montgomery step and one conditional subtraction, in place: the LQ limbs of
+// the result (zero beyond L) end in x[1 .. 1 + LQ).  (x + m q) / 2^32 with
+// m < 2^32 is below 2q, so one subtraction finishes it.  The subtraction runs
+// as a borrow-only pass (r >= q ?) and a masked pass, so no second limb array
 // is live.
 template <int LQ, int NA>
-__device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv) {
-    static_assert(NA >= LQ + 4, "room for the three steps");
-#pragma unroll
-    for (int st = 0; st < 3; ++st) mac_limbs<NA>(x, qc->q, LQ, x[st] * q_inv, st);  // x[st] becomes 0
-    uint32_t br = 0;  // r = x[3 ..] < 2q: borrow of r - q
+__device__ __forceinline__ void redc32(uint32_t (&x)[NA], const QConst* __restrict__ qc, uint32_t q_inv) {
+    static_assert(NA >= LQ + 2, "room for the step and its carry");
+    mac_limbs<NA>(x, qc->q, LQ, x[0] * q_inv, 0);  // x[0] becomes 0
+    uint32_t br = 0;  // r = x[1 ..] < 2q: borrow of r - q
 #pragma unroll
     for (int l = 0; l <= LQ; ++l) {
         uint32_t nb;
-        (void)__builtin_subc(x[3 + l], l < LQ ? qc->q[l] : 0u, br, &nb);
+        (void)__builtin_subc(x[1 + l], l < LQ ? qc->q[l] : 0u, br, &nb);
         br = nb;
     }
     const bool ge = br == 0;  // subtract q iff r >= q
@@ -1194,14 +1194,14 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 #pragma unroll
     for (int l = 0; l < LQ; ++l) {
         uint32_t nb;
-        x[3 + l] = __builtin_subc(x[3 + l], ge ? qc->q[l] : 0u, br, &nb);
+        x[1 + l] = __builtin_subc(x[1 + l], ge ? qc->q[l] : 0u, br, &nb);
         br = nb;
     }
 }
 
 // 30-bit primes on the matrix cores.  The weighted sum is an int8
 // contraction over kappa = 4 j + u (byte u of v_j): table rows hold the bytes
-// of 2^(8u) ((M/p_j) mod q) 2^96 mod q, of 2^(8u) floor(2^56 / p_j) and a row
+// of 2^(8u) ((M/p_j) mod q) 2^32 mod q, of 2^(8u) floor(2^56 / p_j) and a row
 // of ones (layout and offsets: kCrtRowStride and nc_tables.cpp
 // build_crt_mfma).  A wave takes 64 coefficients as two 32-coefficient tiles;
 // the B operand of lane (c, h) is simply the four words v_j of primes
@@ -1210,7 +1210,10 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 // lambda = 8 rt + 2 m + h.  One v_permlane32_swap per word hands lane h all
 // pseudo-limbs of tile h's coefficient c, and every lane finishes one
 // coefficient: carry the 64-bit limb sums, round the fraction to K, add
-// K ((-M) mod q) 2^96, and a Montgomery finish removes the 2^96.
+// K ((-M) mod q) 2^32, and a Montgomery step removes the 2^32: the sum is a
+// byte-weighted sum sum_kappa b_kappa m_kappa of at most 256 terms b < 2^8,
+// m < q, so with the K term it stays below (2^16 + 2^7) q, well inside the
+// single step's range.
 // KCT > 0 fixes the number of contraction chunks at compile time (kc_n ==
 // KCT); KCT == 0 takes kc_n <= kCrtMaxKC at run time.  LT > 0 fixes L.
 // A wave's 64 results (64 L contiguous words of out) leave through its own
@@ -1226,7 +1229,7 @@ __device__ __forceinline__ void redc96(uint32_t (&x)[NA], const QConst* __restri
 // residues enter the contraction lazy, in [0, 2p) rather than canonical:
 // the sum x = sum_j v_j (M / p_j) - K M is the same integer for any
 // representatives, and the bounds hold for every v_j < 2^31 -- F < 2^63,
-// K = round(F / 2^56) <= 127, the limb sum below 2^(32 L + 17) and x < q 2^96
+// K = round(F / 2^56) <= 127, the limb sum below 2^(32 L + 17) and x < 2^17 q
 // for the Montgomery finish -- which spares one conditional subtraction per
 // residue (6-7 % of the kernel's time).
 // One prime's constants for the fused inverse stages as one half of a
@@ -1432,7 +1435,7 @@ __device__ __forceinline__ void crt30_mfma_group(const Lds& s, uint32_t g0, cons
     auto ysum = [&](int lam, int32_t lo, int32_t hi) -> int64_t {
         return (int64_t)lo + (int64_t)((uint64_t)(int64_t)hi << 16) + sbterm + corr[lam];
     };
-    constexpr int NA = LQ + 4;
+    constexpr int NA = LQ + 2;
     uint32_t x[NA];
     int64_t carry = 0;
 #pragma unroll
@@ -1465,8 +1468,8 @@ __device__ __forceinline__ void crt30_mfma_group(const Lds& s, uint32_t g0, cons
     const uint64_t F = (uint64_t)last_sum(LF) + ((uint64_t)last_sum(LF + 1) << 32);
     const uint32_t K = (uint32_t)(F >> 56) + (uint32_t)((F >> 55) & 1u);
     mac_limbs<NA>(x, mc->neg_m_r, LQ, K, 0);
-    redc96<LQ, NA>(x, qc, q_inv);
-    const uint32_t* r = x + 3;
+    redc32<LQ, NA>(x, qc, q_inv);
+    const uint32_t* r = x + 1;
     // lane (c, h) holds coefficient g0 + 32 h + c = g0 + lane
     uint32_t* st = const_cast<uint32_t*>(s.out) + wave * (uint32_t)STAGE * L;
     if constexpr (STAGE == 64) {
@@ -2333,7 +2336,7 @@ __global__ __launch_bounds__(64) void crt_products_kernel(const uint64_t* __rest
 
 // One column kappa = 4 j + u of the int8 MFMA CRT table (layout, offsets and
 // meaning as in nc_tables.cpp build_crt_mfma): q-limb rows hold the bytes of
-// m_ju = 2^(8u) ((M / p_j) mod q) 2^96 mod q, fraction rows the bytes of
+// m_ju = 2^(8u) ((M / p_j) mod q) 2^32 mod q, fraction rows the bytes of
 // 2^(8u) floor(2^56 / p_j), ones rows 1 at byte 0; raw bytes are stored
 // offset by -128; columns of j >= k hold raw zeros.  miq: [k][kMaxLimbs].
 __device__ __forceinline__ uint32_t crt_row_of(uint32_t lam, uint32_t b) {
@@ -2355,7 +2358,7 @@ __global__ __launch_bounds__(64) void crt_mfma_table_kernel(const uint32_t* __re
         m[l] = j < k ? miq[(size_t)j * kMaxLimbs + l] : 0u;
     }
     if (j < k) {
-        for (uint32_t d = 0; d < 96 + 8 * u; ++d) dbl_mod(m, qq, L);
+        for (uint32_t d = 0; d < 32 + 8 * u; ++d) dbl_mod(m, qq, L);
         g = ((uint64_t)1 << 56) / primes[j];
     }
     for (uint32_t lam = 0; lam < lq; ++lam)
@@ -2373,7 +2376,7 @@ __global__ __launch_bounds__(64) void crt_mfma_table_kernel(const uint32_t* __re
 
 // Per pseudo-limb corrections sum_b 2^(8b) (128 SA - 128^2 Kdim), SA the
 // raw-byte row sums (q-limb and fraction rows), and, in thread 8 rt, the
-// Montgomery-finish constant ((-M) mod q) 2^96 mod q from neg_m.
+// Montgomery-finish constant ((-M) mod q) 2^32 mod q from neg_m.
 __global__ __launch_bounds__(64) void crt_mfma_corr_kernel(const int8_t* __restrict__ tab, uint32_t rt, uint32_t kc,
                                                            const uint32_t* __restrict__ neg_m,
                                                            const uint32_t* __restrict__ q, uint32_t L,
@@ -2398,7 +2401,7 @@ __global__ __launch_bounds__(64) void crt_mfma_corr_kernel(const int8_t* __restr
             qq[l] = q[l];
             x[l] = neg_m[l];
         }
-        for (uint32_t d = 0; d < 96; ++d) dbl_mod(x, qq, L);
+        for (uint32_t d = 0; d < 32; ++d) dbl_mod(x, qq, L);
         for (uint32_t l = 0; l < (uint32_t)kMaxLimbs; ++l) neg_m_r[l] = l < L ? x[l] : 0u;
     }
 }

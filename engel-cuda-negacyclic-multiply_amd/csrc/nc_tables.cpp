@@ -249,7 +249,7 @@ void build62(HostTables& t, const Big& qb, const DeviceInit* init) {
 
 // int8 MFMA CRT table.  With kappa = 4 j + u and raw bytes A[row][kappa]:
 //   q-limb rows (lambda < LQ, byte b): byte 4 lambda + b of
-//       m_ju = (2^(8u) ((M / p_j) mod q) R) mod q, R = 2^96, so that
+//       m_ju = (2^(8u) ((M / p_j) mod q) R) mod q, R = 2^32, so that
 //       sum_{lambda, b} 2^(32 lambda + 8 b) X[lambda, b] = sum_ju v_ju m_ju
 //       = R sum_j v_j ((M / p_j) mod q)  (mod q),  v_ju byte u of v_j
 //       (the kernel's Montgomery finish removes R);
@@ -267,7 +267,8 @@ void build62(HostTables& t, const Big& qb, const DeviceInit* init) {
 void build_crt_mfma(HostTables& t, const Big& qb, const DeviceInit* init) {
     HostSet30& s = t.s30;
     const uint32_t L = t.L, k = (uint32_t)s.primes.size();
-    // sum_t 2^(8t) X_t < 2^(32 L + 17) must stay below the Barrett range 2^(bits(q) + 71)
+    // built for moduli that fill their L limbs to within 54 bits; narrower
+    // ones (a short q padded to many limbs) keep the VALU CRT
     if (t.q_bits + 54 < 32 * L) return;
     const uint32_t rt = (L + 4 + 7) / 8, kc = (k + 7) / 8;
     const uint32_t lf = 8 * rt - 4, lq = std::min(lf, (uint32_t)kMaxLimbs);
@@ -295,10 +296,10 @@ void build_crt_mfma(HostTables& t, const Big& qb, const DeviceInit* init) {
         sa[row] += raw;
     };
     for (uint32_t j = 0; j < 8 * kc; ++j) {
-        Big m(L + 4);
+        Big m(L + 2);
         uint64_t g = 0;
         if (j < k) {
-            for (uint32_t l = 0; l < L; ++l) m.v[l + 3] = s.miq[(size_t)j * kMaxLimbs + l];  // miq_j R
+            for (uint32_t l = 0; l < L; ++l) m.v[l + 1] = s.miq[(size_t)j * kMaxLimbs + l];  // miq_j R
             m = mod(m, qb);
             m.v.resize(L + 1, 0u);
             g = (uint64_t)((((u128)1) << 56) / s.primes[j]);
@@ -326,8 +327,8 @@ void build_crt_mfma(HostTables& t, const Big& qb, const DeviceInit* init) {
         s.crt_c[lam] = c;
     }
     // Montgomery finish constants: ((-M) mod q) R mod q and -q^-1 mod 2^32
-    Big nm(L + 3);
-    for (uint32_t l = 0; l < L; ++l) nm.v[l + 3] = s.qc.neg_m[l];
+    Big nm(L + 1);
+    for (uint32_t l = 0; l < L; ++l) nm.v[l + 1] = s.qc.neg_m[l];
     nm = mod(nm, qb);
     std::memset(&s.crt_mc, 0, sizeof s.crt_mc);
     for (uint32_t l = 0; l < L; ++l) s.crt_mc.neg_m_r[l] = nm.v[l];
@@ -658,12 +659,51 @@ void check_crt_mfma(const HostTables& t, std::mt19937_64& rng) {
                 c = x >> 32;
             }
         }
-        Big wr(L + 6);  // want R
-        for (uint32_t l = 0; l < L + 3; ++l) wr.v[l + 3] = want.v[l];
+        Big wr(L + 4);  // want R
+        for (uint32_t l = 0; l < L + 3; ++l) wr.v[l + 1] = want.v[l];
         Big got(lq + 2);
         got.v = acc;
         if (cmp(mod(got, qb), mod(wr, qb)) != 0) throw std::runtime_error("self test: CRT table sum");
-        if (got.bits() > (int)t.q_bits + 71) throw std::runtime_error("self test: CRT table range");
+        // the kernel adds K ((-M) mod q) R mod q (K < 128) and runs one 2^32
+        // Montgomery step, whose result is below 2q only for sums below
+        // (2^32 - 128) q: the byte-weighted sum stays below 2^16 q
+        if (got.bits() > (int)t.q_bits + 16) throw std::runtime_error("self test: CRT table range");
+        // the kernel's finish on these limbs: K = round(F / 2^56), x = sum +
+        // K neg_m_r, one 2^32 Montgomery step, one conditional subtraction;
+        // the result must be c mod q, c = sum_j v_j (M / p_j) - K M, i.e.
+        // (sum_j v_j ((M / p_j) mod q) + K ((-M) mod q)) mod q
+        const uint32_t K = (uint32_t)(fwant >> 56) + (uint32_t)((fwant >> 55) & 1u);
+        std::vector<uint32_t> x = acc;
+        uint64_t cy = 0;
+        for (uint32_t l = 0; l < lq + 2; ++l) {
+            const uint64_t z = (uint64_t)x[l] + (l < L ? (uint64_t)K * s.crt_mc.neg_m_r[l] : 0u) + cy;
+            x[l] = (uint32_t)z;
+            cy = z >> 32;
+        }
+        const uint32_t mq = x[0] * s.crt_mc.q_inv;
+        for (uint32_t l = 0; l < lq + 2; ++l) {
+            const uint64_t z = (uint64_t)x[l] + (l < L ? (uint64_t)mq * s.qc.q[l] : 0u) + cy;
+            x[l] = (uint32_t)z;
+            cy = z >> 32;
+        }
+        if (cy || x[0]) throw std::runtime_error("self test: CRT finish step");
+        Big r(lq + 1), q2(lq + 1);
+        for (uint32_t l = 0; l <= lq; ++l) r.v[l] = x[l + 1];
+        for (uint32_t l = 0; l < L; ++l) q2.v[l] = s.qc.q[l];
+        q2.mul_u64(2);
+        if (cmp(r, q2) >= 0) throw std::runtime_error("self test: CRT finish range");
+        if (cmp(r, qb) >= 0) sub_in(r, qb);
+        Big kn(L + 3);
+        for (uint32_t l = 0; l < L; ++l) kn.v[l] = s.qc.neg_m[l];
+        kn.mul_u64(K);
+        uint64_t c2 = 0;
+        for (uint32_t l = 0; l < L + 3; ++l) {
+            const uint64_t z = (uint64_t)want.v[l] + kn.v[l] + c2;
+            want.v[l] = (uint32_t)z;
+            c2 = z >> 32;
+        }
+        if (c2 || cmp(mod(want, qb), mod(r, qb)) != 0 || r.bits() > (int)t.q_bits)
+            throw std::runtime_error("self test: CRT finish value");
     }
 }
 
