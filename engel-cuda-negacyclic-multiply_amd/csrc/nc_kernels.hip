@@ -1584,7 +1584,8 @@ __global__ __launch_bounds__(256) void crt30_mfma_generic_kernel(
 // the D items write them as whole 128-byte lines, so any copy its own stores
 // left in L1 is current.  A C item's rows were rewritten in place by T items,
 // perhaps on other compute units after this one's D item wrote them, so it
-// first invalidates its L1 (buffer_inv sc0).  Each wait is bounded in time
+// first invalidates its L1 with an agent-scope acquire (buffer_inv sc1).
+// Each wait is bounded in time
 // (s_memrealtime, 100 MHz) and by an iteration cap: on expiry the workgroup
 // sets the XCD's abort flag and error bits and the plan's error word, and
 // every workgroup of that XCD stops, so the launch always ends.
@@ -1688,7 +1689,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
                     } else if (kind == kItemC) {
                         // a C item's rows were rewritten in place by T items
                         // on other compute units: drop this one's L1 copies
-                        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+                        // (an agent-scope acquire, buffer_inv sc1; the
+                        // workgroup-scope form does not reach other units'
+                        // writes)
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     }
                 }
                 __syncthreads();
