@@ -54,13 +54,17 @@ struct Tw {
 // ---- 30-bit prime set (p < 2^30, 32-bit words) ----
 constexpr int kMaxPrimes30 = 72;
 
+// The 30-bit constants are kMergeVariants30 blocks of k rows: block v has nr
+// and wr times 2^v, for inner blocks that merge v levels of the transform
+// into the pointwise product (their output is 2^-v scaled).
 struct PrimeConst32 {
     uint32_t p, p2;
     uint32_t mont_neg;  // -p^-1 mod 2^32
-    uint32_t nr, nr_s;  // N^-1 2^32 mod p, Shoup companion floor(nr 2^32 / p)
-    uint32_t wr, wr_s;  // psi^-bitrev(1) N^-1 2^32 mod p
+    uint32_t nr, nr_s;  // N^-1 2^32 (2^v) mod p, Shoup companion floor(nr 2^32 / p)
+    uint32_t wr, wr_s;  // psi^-bitrev(1) N^-1 2^32 (2^v) mod p
     uint32_t c32;       // 2^32 mod p (decomposition fold constant)
 };
+constexpr int kMergeVariants30 = 3;
 
 struct Tw32 {
     uint32_t w;

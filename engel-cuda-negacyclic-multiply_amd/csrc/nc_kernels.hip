@@ -467,6 +467,18 @@ struct InnerShape {
     static constexpr int PADB = (1 << LOGB) + (1 << (LOGB - 4));  // one pad word per 16
 };
 
+// Levels the 30-bit inner block merges (ntt_inner30_block, merge_product30):
+// its last MG forward stages, the pointwise product and its first MG inverse
+// stages become one product mod X^(2^MG) - C per group of 2^MG adjacent
+// positions, which leaves the block's output scaled by 2^-MG; the prime
+// constants of variant MG (rows MG k .. MG k + k - 1 of the 30-bit
+// PrimeConst32 table) carry 2^MG in the final inverse stage's factors.  Two
+// levels where the last pass holds two stages or more, one where it holds
+// one; blocks below 2^5 take the one-operand kernel and merge nothing.
+__host__ __device__ constexpr uint32_t merge30(uint32_t logb) {
+    return logb < 5 ? 0u : (logb - 4 * ((logb - 1) / 4) >= 2 ? 2u : 1u);
+}
+
 // Layout of pass m: register e of thread tid holds local index
 //   x = base(tid) + delta(e),  delta(e) = e << LO,
 // where base fills the other bits.  Pass m < NP - 1 covers the four bits
@@ -641,17 +653,23 @@ __global__ __launch_bounds__(InnerShape<LOGB>::WG) void ntt_inner_kernel(
 // pass of R < 4 bits has spare register bits above its own, see ct_pass),
 // loaded ahead of the pass: before the LDS exchange that precedes it, so
 // their latency hides behind the exchange.  A top stage's slot reads tw[1]
-// (in bounds, unused).
+// (in bounds, unused).  Slots of stages a pass does not run stay unloaded
+// (and take no registers); the last forward pass also holds the merge
+// stage's twiddles in slot MG (merge_product30).
 template <int R>
 struct PassTw30 {
-    Tw32 t[R][8];
+    Tw32 t[4][8];
 };
 
-template <int R>
+// Stages kk in [KMIN, R) of a pass, and stage KX >= R as well when KX >= 0
+// (the merge stage of a last pass that does not run it).
+template <int R, int KMIN = 0, int KX = -1>
 __device__ __forceinline__ void load_pass_tw(PassTw30<R>& pt, uint32_t gx0, int lo, int log_n,
                                              const Tw32* __restrict__ tw) {
+    static_assert(KX < 4, "four stage slots");
 #pragma unroll
-    for (int kk = R - 1; kk >= 0; --kk) {
+    for (int kk = 3; kk >= KMIN; --kk) {
+        if (kk >= R && kk != KX) continue;
         const int beta = lo + kk;
         const uint32_t base = (1u << (log_n - 1 - beta)) + (gx0 >> (beta + 1));
 #pragma unroll
@@ -659,11 +677,11 @@ __device__ __forceinline__ void load_pass_tw(PassTw30<R>& pt, uint32_t gx0, int 
     }
 }
 
-template <class A, int R>
+template <class A, int R, int KMIN = 0>
 __device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A::T (&vb)[16], const PassTw30<R>& pt,
                                              typename A::T p, typename A::T p2) {
 #pragma unroll
-    for (int kk = R - 1; kk >= 0; --kk) {
+    for (int kk = R - 1; kk >= KMIN; --kk) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             if (e & (1 << kk)) continue;
@@ -673,12 +691,63 @@ __device__ __forceinline__ void ct_pass_pair(typename A::T (&va)[16], typename A
     }
 }
 
-// The inverse pass of 30-bit words on preloaded twiddles (MAY_TOP: as gs_pass).
-template <int R, int NE, bool MAY_TOP>
+// The last MG forward stages of a and b (stages 0 .. MG - 1 of the last pass,
+// whose registers e hold positions x0 + e), the pointwise product and the
+// first MG inverse stages, as one product mod X^G - C (G = 2^MG) per group of
+// G adjacent registers: the stage-MG butterflies leave each group as the
+// operand mod X^G - C, coefficients in register order, with C = w^2 for the
+// group's stage-(MG - 1) twiddle w, i.e. C = +/- t for its stage-MG twiddle t
+// (minus where bit MG of the position is set).  In: a, b in [0, 4p).  Out
+// (va): coefficient k of the product times 2^-32, in [0, 2p) -- 2^-MG times
+// what the separate stages give (the variant-MG prime constants restore it).
+//   a -> [0, p); b -> [0, 2p); C b_j by Shoup on the stored negated twiddle
+//   (+t b = -nv, -t b = nv + 2p, both in [0, 2p]); each coefficient
+//   T = sum_i a_i (b_(k-i) or C b_(k-i+G)) is one 64-bit sum below
+//   2 G p^2 <= 8 p^2, reduced as (T + m p) / 2^32 (m = T (-p^-1) mod 2^32)
+//   < 2 G p^2 / 2^32 + p: below 2p for G = 2 and below 3p for G = 4, which
+//   takes one more conditional subtraction (4p < 2^32).
+template <int MG>
+__device__ __forceinline__ void merge_product30(uint32_t (&va)[16], const uint32_t (&vb)[16], const Tw32 (&mt)[8],
+                                                const PrimeConst32& c) {
+    static_assert(MG == 1 || MG == 2, "one or two merged levels");
+    constexpr int G = 1 << MG;
+    const uint32_t p = c.p, p2 = c.p2, mneg = c.mont_neg;
+#pragma unroll
+    for (int g0 = 0; g0 < 16; g0 += G) {
+        const Tw32 w = mt[g0 >> (MG + 1)];
+        uint32_t a[G], b[G], bc[G];
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            a[i] = W30::sub_if_ge(W30::sub_if_ge(va[g0 + i], p2), p);
+            b[i] = W30::sub_if_ge(vb[g0 + i], p2);
+        }
+#pragma unroll
+        for (int j = 1; j < G; ++j) {
+            const uint32_t nv = __umulhi(vb[g0 + j], w.ws) * p + vb[g0 + j] * w.w;
+            bc[j] = ((g0 >> MG) & 1) ? nv + p2 : 0u - nv;
+        }
+#pragma unroll
+        for (int kx = 0; kx < G; ++kx) {
+            uint64_t t = 0;
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+                const int jj = kx - i;
+                t += (uint64_t)a[i] * (jj >= 0 ? b[jj] : bc[jj + G]);
+            }
+            const uint32_t m = (uint32_t)t * mneg;
+            const uint32_t r = (uint32_t)((t + (uint64_t)m * p) >> 32);
+            va[g0 + kx] = MG == 1 ? r : W30::sub_if_ge(r, p2);
+        }
+    }
+}
+
+// The inverse pass of 30-bit words on preloaded twiddles (MAY_TOP: as
+// gs_pass), from stage KMIN (the first inverse pass skips the merged ones).
+template <int R, int NE, bool MAY_TOP, int KMIN = 0>
 __device__ __forceinline__ void gs_pass_tw(uint32_t (&v)[NE], const PassTw30<R>& pt, int lo, int log_n,
                                            const PrimeConst32& c) {
 #pragma unroll
-    for (int kk = 0; kk < R; ++kk) {
+    for (int kk = KMIN; kk < R; ++kk) {
         const int beta = lo + kk;
         if (MAY_TOP && kk == R - 1 && beta == log_n - 1) {  // stage 0: merged N^-1 R scaling
 #pragma unroll
@@ -718,20 +787,38 @@ __device__ __forceinline__ void exchange_pair(uint32_t (&va)[16], uint32_t (&vb)
 // PF: each pass's twiddles are loaded before the exchange that precedes it
 // (the caller loads pt); otherwise at the start of the pass (pt is loaded
 // here).  Prefetching keeps one more pass's twiddles live across the
-// exchange, which only some block sizes have the registers for.
+// exchange, which only some block sizes have the registers for.  The last
+// pass stops above the merged stages and ends in merge_product30: va then
+// holds the merged product (2^-MG scaled).
+template <int LOGB, int M>
+struct FwdPass30 {
+    static constexpr bool kLast = M + 1 == InnerShape<LOGB>::NP;
+    static constexpr int MG = (int)merge30(LOGB);
+    static constexpr int KMIN = kLast ? MG : 0;  // lowest stage the pass runs
+    static constexpr int KX = kLast ? MG : -1;   // merge twiddles in slot MG
+    using P = PassMap<LOGB, M>;
+    static __device__ __forceinline__ void load(PassTw30<P::R>& pt, uint32_t gblk, uint32_t tid, int log_n,
+                                                const Tw32* tw) {
+        load_pass_tw<P::R, KMIN, KX>(pt, gblk | P::base(tid), P::LO, log_n, tw);
+    }
+};
+
 template <int LOGB, int M, bool PF>
 __device__ __forceinline__ void forward_pair_from(uint32_t (&va)[16], uint32_t (&vb)[16], uint2* lds, uint32_t tid,
-                                                  uint32_t gblk, int log_n, const Tw32* tw, uint32_t p, uint32_t p2,
+                                                  uint32_t gblk, int log_n, const Tw32* tw, const PrimeConst32& c,
                                                   PassTw30<PassMap<LOGB, M>::R>& pt) {
-    using P = PassMap<LOGB, M>;
+    using F = FwdPass30<LOGB, M>;
+    using P = typename F::P;
     if constexpr (M > 0) exchange_pair<LOGB, M - 1, M>(va, vb, lds, tid);
-    if constexpr (!PF) load_pass_tw<P::R>(pt, gblk | P::base(tid), P::LO, log_n, tw);
-    ct_pass_pair<W30, P::R>(va, vb, pt, p, p2);
-    if constexpr (M + 1 < InnerShape<LOGB>::NP) {
-        using Q = PassMap<LOGB, M + 1>;
-        PassTw30<Q::R> nt;
-        if constexpr (PF) load_pass_tw<Q::R>(nt, gblk | Q::base(tid), Q::LO, log_n, tw);
-        forward_pair_from<LOGB, M + 1, PF>(va, vb, lds, tid, gblk, log_n, tw, p, p2, nt);
+    if constexpr (!PF) F::load(pt, gblk, tid, log_n, tw);
+    ct_pass_pair<W30, P::R, F::KMIN>(va, vb, pt, c.p, c.p2);
+    if constexpr (!F::kLast) {
+        using G = FwdPass30<LOGB, M + 1>;
+        PassTw30<G::P::R> nt;
+        if constexpr (PF) G::load(nt, gblk, tid, log_n, tw);
+        forward_pair_from<LOGB, M + 1, PF>(va, vb, lds, tid, gblk, log_n, tw, c, nt);
+    } else {
+        merge_product30<F::MG>(va, vb, pt.t[F::MG], c);
     }
 }
 
@@ -740,8 +827,10 @@ __device__ __forceinline__ void inverse30_from(uint32_t (&v)[16], uint32_t* lds,
                                                int log_n, const Tw32* tw, const PrimeConst32& c,
                                                PassTw30<PassMap<LOGB, M>::R>& pt) {
     using P = PassMap<LOGB, M>;
-    if constexpr (!PF) load_pass_tw<P::R>(pt, gblk | P::base(tid), P::LO, log_n, tw);
-    gs_pass_tw<P::R, 16, M == 0>(v, pt, P::LO, log_n, c);
+    // the first inverse pass (the last pass's layout) starts above the merged stages
+    constexpr int KMIN = M + 1 == InnerShape<LOGB>::NP ? (int)merge30(LOGB) : 0;
+    if constexpr (!PF) load_pass_tw<P::R, KMIN>(pt, gblk | P::base(tid), P::LO, log_n, tw);
+    gs_pass_tw<P::R, 16, M == 0, KMIN>(v, pt, P::LO, log_n, c);
     if constexpr (M > 0) {
         using Q = PassMap<LOGB, M - 1>;
         PassTw30<Q::R> nt;
@@ -753,15 +842,17 @@ __device__ __forceinline__ void inverse30_from(uint32_t (&v)[16], uint32_t* lds,
 
 // One 2^LOGB block of one prime row: ra_blk / rb_blk point at the block in
 // a's and b's rows (the product overwrites ra_blk), gblk is the block's first
-// row index, tf / ti the prime's twiddle rows.  tid: the thread's index in
-// its (sub-)block; lds2: the sub-block's padded region.  Every thread of the
-// workgroup calls this (barriers inside); an inactive sub-block computes and
-// stores nothing it is not given to store.
+// row index, tf / ti the prime's twiddle rows, c the prime's constants of
+// variant merge30(LOGB).  tid: the thread's index in its (sub-)block; lds2:
+// the sub-block's padded region.  Every thread of the workgroup calls this
+// (barriers inside); an inactive sub-block computes and stores nothing it is
+// not given to store.
 template <int LOGB>
 __device__ __forceinline__ void ntt_inner30_block(uint2* lds2, uint32_t tid, uint32_t* __restrict__ ra_blk,
                                                   const uint32_t* __restrict__ rb_blk, uint32_t gblk,
                                                   const PrimeConst32& c, const Tw32* tf, const Tw32* ti,
                                                   uint32_t log_n, bool active, bool final_out) {
+    static_assert(LOGB >= 5, "blocks of 2^5 and up (merge30)");
     using S = InnerShape<LOGB>;
     using T = uint32_t;
     constexpr int LAST = S::NP - 1;
@@ -780,13 +871,12 @@ __device__ __forceinline__ void ntt_inner30_block(uint2* lds2, uint32_t tid, uin
     // 128 VGPRs); smaller blocks load at the start of each pass
     constexpr bool PF = LOGB >= 10;
     PassTw30<P0::R> tw0;
-    if constexpr (PF) load_pass_tw<P0::R>(tw0, gblk | P0::base(tid), P0::LO, (int)log_n, tf);
-    forward_pair_from<LOGB, 0, PF>(va, vb, lds2, tid, gblk, (int)log_n, tf, c.p, c.p2, tw0);
-#pragma unroll
-    for (int e = 0; e < 16; ++e) va[e] = W30::mont(W30::sub_if_ge(va[e], c.p2), W30::sub_if_ge(vb[e], c.p2), c.p, c.mont_neg);
+    if constexpr (PF) FwdPass30<LOGB, 0>::load(tw0, gblk, tid, (int)log_n, tf);
+    // forward stages down to the merged ones, then the merged product
+    forward_pair_from<LOGB, 0, PF>(va, vb, lds2, tid, gblk, (int)log_n, tf, c, tw0);
     using PL = PassMap<LOGB, LAST>;
     PassTw30<PL::R> twl;
-    if constexpr (PF) load_pass_tw<PL::R>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
+    if constexpr (PF) load_pass_tw<PL::R, (int)merge30(LOGB)>(twl, gblk | PL::base(tid), PL::LO, (int)log_n, ti);
     __syncthreads();  // the paired region is reused below with the one-operand layout
     inverse30_from<LOGB, LAST, PF>(va, lds1, tid, gblk, (int)log_n, ti, c, twl);
     if (!active) return;
@@ -1718,13 +1808,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void p
                 if (!active) wid = 0;
                 const uint32_t j = wid >> 2, blk = wid & 3u;
                 const size_t off = ((size_t)P * k + j) * N + ((size_t)blk << LOGB);
-                const PrimeConst32 c = A.w.pc[j];
+                const PrimeConst32 c = A.w.pc[merge30(LOGB) * k + j];  // the block's merge variant
                 ntt_inner30_block<LOGB>(u.t + sub * S::PADB, tid, A.ra + off, A.rb + off, blk << LOGB, c,
                                         A.w.tw_fwd + ((size_t)j << A.w.tw_log),
                                         A.w.tw_inv + ((size_t)j << A.w.tw_log), A.log_n, active, false);
             } else {
                 if (role != kItemC) {
-                    crt30_mfma_tables<4, 8, 28, 2, kPersistCrtPitch>(u.c, A.w.crt_tab, k, 8u, A.w.pc, A.w.tw_inv,
+                    crt30_mfma_tables<4, 8, 28, 2, kPersistCrtPitch>(u.c, A.w.crt_tab, k, 8u,
+                                                                    A.w.pc + merge30(LOGB) * k, A.w.tw_inv,
                                                                     A.w.tw_log);
                     __syncthreads();
                     role = kItemC;
@@ -1879,6 +1970,17 @@ struct SetOf<W30> {
     static const DevSet30& get(const DevPlan& d) { return d.w30; }
 };
 
+// The prime constants the final inverse stage reads after an inner block of
+// 2^logb: for the 30-bit set the variant of that block's merge depth
+// (merge30; 0 for the one-operand and tiny kernels, whose blocks are below
+// 2^5), for the 62-bit set the only one.
+template <class A>
+static const typename A::Prime* pc_for(const DevPlan& d, uint32_t logb) {
+    const auto& set = SetOf<A>::get(d);
+    if constexpr (std::is_same<A, W30>::value) return set.pc + (size_t)merge30(logb) * set.k;
+    return set.pc;
+}
+
 template <class A, int LOGB>
 static void launch_inner(const DevPlan& d, void* ra, const void* rb, uint32_t nprod, hipStream_t s) {
     using S = InnerShape<LOGB>;
@@ -1888,10 +1990,12 @@ static void launch_inner(const DevPlan& d, void* ra, const void* rb, uint32_t np
     const uint32_t grid = (items + S::BPW - 1) / S::BPW;
     if constexpr (std::is_same<A, W30>::value && LOGB >= 5)
         ntt_inner30_kernel<LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                         set.tw_inv, set.pc, d.log_n, set.k, nprod, set.tw_log);
+                                                         set.tw_inv, pc_for<A>(d, LOGB), d.log_n, set.k, nprod,
+                                                         set.tw_log);
     else
         ntt_inner_kernel<A, LOGB><<<grid, S::WG, 0, s>>>(static_cast<T*>(ra), static_cast<const T*>(rb), set.tw_fwd,
-                                                          set.tw_inv, set.pc, d.log_n, set.k, nprod, set.tw_log);
+                                                          set.tw_inv, pc_for<A>(d, LOGB), d.log_n, set.k, nprod,
+                                                          set.tw_log);
 }
 
 template <class A>
@@ -1938,8 +2042,8 @@ static void launch_outer_inv(const DevPlan& d, void* ra, uint32_t nprod, hipStre
     const auto& set = SetOf<A>::get(d);
     const uint32_t wgs_per_row = ((d.N >> S1) + 255) / 256;
     const uint32_t grid = wgs_per_row * set.k * nprod;
-    ntt_outer_inv_kernel<A, S1><<<grid, 256, 0, s>>>(static_cast<T*>(ra), set.tw_inv, set.pc, d.log_n, set.k, nprod,
-                                                      set.tw_log);
+    ntt_outer_inv_kernel<A, S1><<<grid, 256, 0, s>>>(static_cast<T*>(ra), set.tw_inv, pc_for<A>(d, d.log_n - S1),
+                                                      d.log_n, set.k, nprod, set.tw_log);
 }
 
 template <class A>
@@ -2089,8 +2193,8 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     const uint32_t cap = d.cus * (per_cu < 2u ? per_cu : 2u);
     const uint32_t grid = groups < cap ? groups : cap;
     crt30_mfma_kernel<4, 8, 28, 2><<<grid, 256, 0, s>>>(res, out, d.w30.crt_tab, d.w30.crt_c, d.w30.crt_mc, d.w30.qc,
-                                                        d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod, d.w30.pc,
-                                                        d.w30.tw_inv, d.w30.tw_log);
+                                                        d.log_n, d.L, d.w30.k, d.w30.crt_kc, nprod,
+                                                        pc_for<W30>(d, d.log_n - 2), d.w30.tw_inv, d.w30.tw_log);
     return hipGetLastError();
 }
 

@@ -335,6 +335,24 @@ void build_crt_mfma(HostTables& t, const Big& qb, const DeviceInit* init) {
     s.crt_mc.q_inv = (uint32_t)neg_inv(qb.v[0], 32);
 }
 
+// Rows k .. 3k - 1 of the 30-bit prime constants: variant v (1, 2) of row j
+// is row j with the final inverse stage's factors nr, wr times 2^v, for inner
+// blocks that merge v levels of the transform into their pointwise product
+// and so leave their output scaled by 2^-v (merge30 in nc_kernels.hip).
+void add_merge_variants(std::vector<PrimeConst32>& pc, size_t k) {
+    pc.resize((size_t)kMergeVariants30 * k);
+    for (size_t v = 1; v < (size_t)kMergeVariants30; ++v)
+        for (size_t j = 0; j < k; ++j) {
+            PrimeConst32 c = pc[j];
+            const uint64_t p = c.p, f = 1ull << v;
+            c.nr = (uint32_t)mulmod(c.nr, f, p);
+            c.nr_s = (uint32_t)shoup_w(c.nr, p, 32);
+            c.wr = (uint32_t)mulmod(c.wr, f, p);
+            c.wr_s = (uint32_t)shoup_w(c.wr, p, 32);
+            pc[v * k + j] = c;
+        }
+}
+
 void build30(HostTables& t, const Big& qb, const DeviceInit* init) {
     HostSet30& s = t.s30;
     Big M;
@@ -384,6 +402,7 @@ void build30(HostTables& t, const Big& qb, const DeviceInit* init) {
             pl = mulmod(pl, c.c32, p);
         }
     }
+    add_merge_variants(s.pc, k);
     // int8 MFMA decomposition table: T_m = 2^(8m) 2^32 mod p_j written in
     // balanced base-256 digits t_um in [-128, 127] (sum_u 2^(8u) t_um = T_m),
     // which are valid signed int8 as they are.  The kernel contracts them with
@@ -527,6 +546,7 @@ HostTables derive_tables(const HostTables& m, uint32_t N) {
         c.wr = (uint32_t)wr;
         c.wr_s = (uint32_t)shoup_w(wr, p, 32);
     }
+    add_merge_variants(t.s30.pc, m.s30.primes.size());
     return t;
 }
 
