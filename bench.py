@@ -146,21 +146,30 @@ def mixed_batch(q: np.ndarray, n: int, seed: int, launches: int):
     graph_s = (time.perf_counter() - t0) / launches
     got = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
     # the same graph with the persistent form on for every degree it covers
-    # (2^10 .. 2^14 groups of at least one product per XCD)
+    # (2^10 .. 2^14 groups of at least one product per XCD), where the
+    # library has the form switched on
     degs = sorted({N for N, _, _, _, _ in groups})
-    batch.set_persist(degs, 1)
-    pgraph = batch.capture(groups)
-    for _ in range(10):
-        pgraph.launch()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(launches):
-        pgraph.launch()
-    torch.cuda.synchronize()
-    pgraph_s = (time.perf_counter() - t0) / launches
-    pgot = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
-    p_errors = batch.persist_status(degs)
-    pgraph.close()
+    try:
+        batch.set_persist(degs, 1)
+        p_on = True
+    except ncmul.NcError as e:
+        if e.status != ncmul.NC_ERR_UNSUPPORTED:
+            raise
+        p_on = False
+    pgot, p_errors = got, 0
+    if p_on:
+        pgraph = batch.capture(groups)
+        for _ in range(10):
+            pgraph.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            pgraph.launch()
+        torch.cuda.synchronize()
+        pgraph_s = (time.perf_counter() - t0) / launches
+        pgot = {N: ncmul.from_device_u32(c) for N, _, _, _, c in groups}
+        p_errors = batch.persist_status(degs)
+        pgraph.close()
     # plain: one product per call on resident operands, per-degree plans tuned for one product
     plains = {}
     for N, cnt, d_a, d_b, _ in groups:
@@ -188,13 +197,17 @@ def mixed_batch(q: np.ndarray, n: int, seed: int, launches: int):
         "products_per_s": n / graph_s, "ms_per_launch": graph_s * 1e3,
         "plain_products_per_s": n / plain_s, "over_plain": plain_s / graph_s,
         "bit_exact_vs_plain": bool(exact and p_errors == 0 and all(np.array_equal(pgot[N], got[N]) for N in got)),
-        "persistent_graph": {"products_per_s": n / pgraph_s, "ms_per_launch": pgraph_s * 1e3,
-                             "wait_errors": p_errors,
-                             "same_output": bool(all(np.array_equal(pgot[N], got[N]) for N in got))},
+        "persistent_graph": ({"products_per_s": n / pgraph_s, "ms_per_launch": pgraph_s * 1e3,
+                              "wait_errors": p_errors,
+                              "same_output": bool(all(np.array_equal(pgot[N], got[N]) for N in got))}
+                             if p_on else {"disabled": PERSIST_OFF}),
     }
 
 
 SERVE_DEGREES = [1 << e for e in range(11, 17)]
+
+PERSIST_OFF = ("the persistent batched form is switched off in this library build: its launches returned "
+               "wrong products now and then for a cause not yet found; every call runs the launch sequence")
 
 
 class ServeMix:
@@ -574,8 +587,13 @@ def main() -> int:
     # ---- the persistent batched form: the whole step in one launch whose
     # workgroups drain per-XCD queues of decomposition, NTT and CRT items ----
     pp = ncmul.Plan(N, q)
-    pp.set_persist(1)
-    persistent = {"available": pp.persistent(B), "chosen_by_autotuner": batched.persistent(B)}
+    try:
+        pp.set_persist(1)
+        persistent = {"available": pp.persistent(B), "chosen_by_autotuner": batched.persistent(B)}
+    except ncmul.NcError as e:
+        if e.status != ncmul.NC_ERR_UNSUPPORTED:
+            raise
+        persistent = {"available": False, "chosen_by_autotuner": batched.persistent(B), "disabled": PERSIST_OFF}
     if persistent["available"]:
         for _ in range(5):
             pp.multiply_device(d_a, d_b, d_c, B)

@@ -393,6 +393,7 @@ const char* nc_status_string(int status) {
         case NC_ERR_SELFTEST: return "init self test failed";
         case NC_ERR_CAPACITY: return "capacity exceeded";
         case NC_ERR_STATE: return "invalid state";
+        case NC_ERR_UNSUPPORTED: return "switched off in this build";
         default: return "unknown status";
     }
 }
@@ -968,6 +969,8 @@ int nc_plan_set_mfma(nc_plan* p, int mode) {
 int nc_plan_set_persist(nc_plan* p, int mode, uint32_t wait_bound_us) {
     if (!p) return fail(NC_ERR_INVALID, "null plan");
     if (mode < -1 || mode > 1) return fail(NC_ERR_INVALID, "mode must be -1, 0 or 1");
+    if (mode == 1 && !nc::persist_launch_enabled())
+        return fail(NC_ERR_UNSUPPORTED, "the persistent batched form is switched off in this build");
     p->persist_mode = mode;
     p->persist_tmo = wait_bound_us ? (uint64_t)wait_bound_us * 100u : 100000000ull;  // 100 MHz ticks
     return NC_OK;

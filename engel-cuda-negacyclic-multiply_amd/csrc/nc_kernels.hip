@@ -2198,8 +2198,20 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     return hipGetLastError();
 }
 
+// The persistent form is switched off: its launches returned wrong products
+// now and then (three of 128 in one of three repeated launches of a
+// 128-product batch; one of three 256-product launches differing from the
+// launch sequence), with no readiness-wait error, and again after the CRT
+// items' L1 invalidate was corrected to agent scope.  Reading the hand-off
+// protocol found no further cause, so the kernel stays in the source and its
+// schedule stays checked on the CPU (persist_emulate), but nothing launches
+// it: every call runs the launch sequence.
+constexpr bool kPersistLaunchEnabled = false;
+
+bool persist_launch_enabled() { return kPersistLaunchEnabled; }
+
 bool persist_available(const DevPlan& d, uint32_t batch) {
-    return d.xcds >= 1 && d.xcds <= kPersistMaxXcd && batch >= d.xcds && d.log_n >= 10 && d.log_n <= 14 &&
+    return kPersistLaunchEnabled && d.xcds >= 1 && d.xcds <= kPersistMaxXcd && batch >= d.xcds && d.log_n >= 10 && d.log_n <= 14 &&
            crt_fuses_outer_inv(d, 30, true, 2) && decompose_fuses_outer(d, 30, true, 2) && d.w30.k <= 64 &&
            persist_shape_for(d.log_n, d.w30.k).R <= kPersistMaxR;
 }
@@ -2212,6 +2224,7 @@ size_t persist_ctl_bytes(const DevPlan& d, uint32_t batch) {
 hipError_t launch_persist30(const DevPlan& d, const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t* ra,
                             uint32_t* rb, uint32_t batch, uint32_t* ctl, uint32_t* sticky, uint64_t tmo,
                             hipStream_t s) {
+    if (!kPersistLaunchEnabled) return hipErrorNotSupported;
     if (!persist_available(d, batch)) return hipErrorInvalidValue;
     const PersistShape sh = persist_shape_for(d.log_n, d.w30.k);
     PersistArgs A{};

@@ -21,9 +21,11 @@ LIB_PATH = Path(os.environ["NC_LIB_PATH"]) if os.environ.get("NC_LIB_PATH") else
 HEADER = PKG_DIR.parent / "include" / "negacyclic.h"
 
 NC_OK = 0
+NC_ERR_UNSUPPORTED = -9  # a form switched off in this build (nc_plan_set_persist mode 1)
 STATUS = {
     0: "NC_OK", -1: "NC_ERR_INVALID", -2: "NC_ERR_SIZE", -3: "NC_ERR_DEVICE", -4: "NC_ERR_NOMEM",
     -5: "NC_ERR_PRIMES", -6: "NC_ERR_SELFTEST", -7: "NC_ERR_CAPACITY", -8: "NC_ERR_STATE",
+    -9: "NC_ERR_UNSUPPORTED",
 }
 
 MFMA_DECOMPOSE = 1  # int8 matrix-core paths of the 30-bit prime set (nc_plan_set_mfma mask)

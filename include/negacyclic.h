@@ -52,6 +52,7 @@ extern "C" {
 #define NC_ERR_SELFTEST -6   /* an init-time known-answer test failed        */
 #define NC_ERR_CAPACITY -7   /* request exceeds a configured capacity        */
 #define NC_ERR_STATE -8      /* call not valid in the object's current state */
+#define NC_ERR_UNSUPPORTED -9 /* the requested form is switched off in this build */
 
 typedef struct nc_plan nc_plan;
 
@@ -180,6 +181,10 @@ int nc_plan_set_mfma(nc_plan* plan, int mode);
  * wait inside the launch (0: 1 s); a wait that reaches it stops its XCD's
  * queue, so the launch still ends, and sets the plan's error word
  * (nc_plan_persist_status).  Output bits never depend on the mode.
+ * The form is switched off in this build (mode 1 returns
+ * NC_ERR_UNSUPPORTED; -1 and 0 are accepted and run the launch sequence):
+ * its launches returned wrong products now and then, for a cause not yet
+ * found.
  * (No counterpart in the reference.) */
 int nc_plan_set_persist(nc_plan* plan, int mode, uint32_t wait_bound_us);
 /* *persistent = 1 if calls with `batch` products run the persistent form. */

@@ -17,7 +17,14 @@ import pytest
 import ncmul
 import oracle as O
 
-pytestmark = pytest.mark.gpu
+# The library has the persistent form switched off (nc_plan_set_persist mode
+# 1 returns NC_ERR_UNSUPPORTED): its launches returned wrong products now and
+# then -- three of 128 in one of three repeated launches of the 128-product
+# batch, one of three 256-product bench launches -- for a cause not yet found.
+# These tests stay as they are, skipped, until the form is switched on again;
+# tests/test_gpu_persist_off.py checks the switch.
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skip(reason="persistent form switched off: intermittent wrong products, cause not found")]
 
 L = 28
 
