@@ -1324,11 +1324,11 @@ __device__ __forceinline__ void redc32(uint32_t (&x)[NA], const QConst* __restri
 // residue (6-7 % of the kernel's time).
 // One prime's constants for the fused inverse stages as one half of a
 // column group sees them (entry 2 r + u, u = 1 for lanes c with c & 16):
-// the span-N/4 twiddle psi^-bitrev(2 + u), the stage-0 factor (N^-1 R y_j
-// for the sum, times psi^-bitrev(1) for the difference) and sadd (0 or
-// 2p + 1), so that no lane selects between constants.
+// the span-N/4 twiddle psi^-bitrev(2 + u) and the stage-0 factors, N^-1 R y_j
+// for the sum and that times psi^-bitrev(1) for the difference (each with its
+// Shoup quotient), so that no lane selects between constants.
 struct CrtInvLds {
-    uint32_t p, w, ws, wf, wfs, sadd, pad0, pad1;
+    uint32_t p, w, ws, nr, nr_s, wr, wr_s, pad;
 };
 
 // LDS of the matrix-core CRT: the int8 table (row pitch PITCH bytes), the
@@ -1370,8 +1370,7 @@ __device__ __forceinline__ void crt30_mfma_tables(Lds& s, const int8_t* __restri
             const uint32_t j = r < k ? r : k - 1;
             const PrimeConst32 c = pcs[j];
             const Tw32 t = twi[((size_t)j << tw_log) + 2 + u];
-            s.inv[r2] = u ? CrtInvLds{c.p, t.w, t.ws, c.wr, c.wr_s, 2 * c.p + 1, 0u, 0u}
-                          : CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, 0u, 0u, 0u};
+            s.inv[r2] = CrtInvLds{c.p, t.w, t.ws, c.nr, c.nr_s, c.wr, c.wr_s, 0u};
         }
     }
 }
@@ -1452,7 +1451,6 @@ __device__ __forceinline__ void crt30_mfma_group(const Lds& s, uint32_t g0, cons
         // out of the group loop (32 primes x 9 words would not fit).
         asm volatile("" ::: "memory");
         const uint32_t upper = (col >> 4) & 1u;
-        const uint32_t m = 0u - upper;  // all ones in the upper half
 #pragma unroll
         for (int kc = 0; kc < KCM; ++kc) {
             if ((uint32_t)kc >= kc_n) continue;  // wave-uniform
@@ -1463,15 +1461,21 @@ __device__ __forceinline__ void crt30_mfma_group(const Lds& s, uint32_t g0, cons
                 // span N/4: tile 0 (x) with tile 1 (x + N/4)
                 uint32_t a = (uint32_t)bf[0][kc][e], b = (uint32_t)bf[1][kc][e];
                 Bfly<W30>::gs(a, b, Tw32{ci.w, ci.ws}, p, p2);
-                // span N/2 (stage 0): lane c pairs with lane c ^ 16; the
-                // lower half forms a + pa, the upper pa - a + 2p, both as
-                // pa + (a ^ m) + sadd
-                const uint32_t pa = (uint32_t)__shfl_xor((int)a, 16);
-                const uint32_t pb = (uint32_t)__shfl_xor((int)b, 16);
-                const uint32_t sa = pa + (a ^ m) + ci.sadd;
-                const uint32_t sb = pb + (b ^ m) + ci.sadd;
-                bf[0][kc][e] = (int)Bfly<W30>::shoup(sa, ci.wf, ci.wfs, p);
-                bf[1][kc][e] = (int)Bfly<W30>::shoup(sb, ci.wf, ci.wfs, p);
+                // span N/2 (stage 0) pairs lane c with lane c ^ 16 (rows of
+                // 16 lanes: row 2i holds the lower positions, row 2i + 1 the
+                // upper).  v_permlane16_swap(a, b) exchanges a's odd rows
+                // with b's even rows, so every lane then holds the lower (lo)
+                // and upper (hi) value of one tile's pair -- tile 0 in even
+                // rows, tile 1 in odd rows -- and forms both outputs; the
+                // second swap sends the sums to the even rows and the
+                // differences to the odd rows, for both tiles.
+                const auto lh = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+                const uint32_t lo = lh[0], hi = lh[1];  // [0, 2p) each
+                const uint32_t sum = Bfly<W30>::shoup(lo + hi, ci.nr, ci.nr_s, p);
+                const uint32_t dif = Bfly<W30>::shoup(lo - hi + p2, ci.wr, ci.wr_s, p);
+                const auto sd = __builtin_amdgcn_permlane16_swap(sum, dif, false, false);
+                bf[0][kc][e] = (int)sd[0];
+                bf[1][kc][e] = (int)sd[1];
             }
         }
     }
