@@ -780,6 +780,18 @@ void self_test(const HostTables& t) {
              t.s30.tw_inv[(size_t)j * t.N + 1].w != t.s30.tw_pow[((size_t)j * 2 + 1) * t.log_n + top].w))
             throw std::runtime_error("self test: twiddle table");
         if ((uint32_t)(p * (0u - c.mont_neg)) != 1u) throw std::runtime_error("self test: Montgomery constant");
+        // the merge variants (rows v k + j): the same prime, the final stage's
+        // factors times 2^v, each Shoup quotient matching its factor
+        const size_t k30 = t.s30.primes.size();
+        if (t.s30.pc.size() != (size_t)kMergeVariants30 * k30) throw std::runtime_error("self test: merge variants");
+        for (int v = 1; v < kMergeVariants30; ++v) {
+            const PrimeConst32& cv = t.s30.pc[(size_t)v * k30 + j];
+            if (cv.p != p || cv.p2 != c.p2 || cv.mont_neg != c.mont_neg || cv.c32 != c.c32 ||
+                mulmod(cv.nr, p - (p - 1) / 2, p) != mulmod(c.nr, 1ull << (v - 1), p) ||
+                mulmod(cv.wr, p - (p - 1) / 2, p) != mulmod(c.wr, 1ull << (v - 1), p) ||
+                cv.nr_s != (uint32_t)shoup_w(cv.nr, p, 32) || cv.wr_s != (uint32_t)shoup_w(cv.wr, p, 32))
+                throw std::runtime_error("self test: merge variants");
+        }
         for (int trial = 0; trial < 16; ++trial) {
             const uint32_t a = (uint32_t)rng(), w = (uint32_t)(rng() % p), ws = (uint32_t)shoup_w(w, p, 32);
             const uint32_t lazy = a * w - (uint32_t)(((uint64_t)a * ws) >> 32) * p;
