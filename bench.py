@@ -426,6 +426,8 @@ def main() -> int:
     ap.add_argument("--plain-calls", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-zero-operands", action="store_true",
+                    help="skip the all-zero-operand leg (profiler runs: every batched launch then sees random data)")
     ap.add_argument("--mixed-products", type=int, default=256, help="mixed-degree graph batch size (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=10000, help="served requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=6000.0, help="Poisson arrival rate, requests/s")
@@ -584,6 +586,32 @@ def main() -> int:
         "same_output_as_30_bit": bool(np.array_equal(c62, c_batch)),
     }
 
+    # ---- the same step on all-zero operands: the kernels issue the same
+    # instructions whatever the data, so any difference is the clock the
+    # GPU holds under the power its multipliers draw on random words ----
+    operand_power = None
+    if not args.no_zero_operands:
+        z_a, z_b = torch.zeros_like(d_a), torch.zeros_like(d_b)
+        z_c = torch.empty_like(d_c)
+        for _ in range(10):
+            batched.multiply_device(z_a, z_b, z_c, B)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batched.multiply_device(z_a, z_b, z_c, B)
+        torch.cuda.synchronize()
+        zero_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        zero_stage_ms = batched.profile_stages(z_a, z_b, z_c, B, reps=args.steps, stream=stream)
+        zero_ok = not bool(z_c.any().item())  # 0 * 0 = 0 for every coefficient
+        del z_a, z_b, z_c
+        operand_power = {
+            "zero_operands_ms_per_step": zero_ms,
+            "random_operands_ms_per_step": elapsed / args.steps * 1e3,
+            "random_over_zero": (elapsed / args.steps * 1e3) / zero_ms,
+            "zero_operands_stage_ms": {k: v for k, v in zero_stage_ms.items() if v > 0},
+            "zero_products_exact": zero_ok,
+        }
+
     # ---- the persistent batched form: the whole step in one launch whose
     # workgroups drain per-XCD queues of decomposition, NTT and CRT items ----
     pp = ncmul.Plan(N, q)
@@ -729,6 +757,7 @@ def main() -> int:
         "roofline_hbm": roofline_hbm,
         "stage_roofline": stage_roofline,
         "width62": width62,
+        "operand_power": operand_power,
         "persistent": persistent,
         "w_convention": {
             "W_per_product": W,

@@ -148,6 +148,11 @@ constexpr uint32_t kHostXferMax = 64;  // buffers per launch
 hipError_t launch_host_xfer(const void* const* src, void* const* dst, uint32_t count, uint64_t bytes, bool to_host,
                             hipStream_t s);
 
+// n limbs of pseudo-random operands for timing runs: coefficients of L limbs
+// below q, given q's highest non-zero limb index hi < L and its value top.
+hipError_t launch_fill_operands(uint32_t* out, uint64_t n, uint32_t L, uint32_t hi, uint32_t top, uint64_t seed,
+                                hipStream_t s);
+
 // 32x32->64 multiply throughput probe (blocks x 256 threads x iters x 8 multiplies).
 hipError_t launch_mul32_peak(const uint32_t* in, uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t s);
 
