@@ -223,11 +223,12 @@ bool persist_drained(nc_plan* p, uint32_t batch, hipStream_t s) {
 
 // ---- persisted autotuner choices ------------------------------------------
 // One text line per tuned (device, shape, batch bucket, pinned settings):
-//   nc-tune-v2 <arch> <cus> <N> <L> <q_bits> <k62> <k30> <bucket> <mfma_mode>
+//   nc-tune-v3 <arch> <cus> <N> <L> <q_bits> <k62> <k30> <bucket> <mfma_mode>
 //   <chunks_forced> <persist_mode> : <width> <log_blocks> <mfma> <chunks>
 //   <persist> <width1> <log_blocks1> <mfma1> <persist1> <median_ms>
 // (the second group is the unchunked choice).  The file is appended to; the
-// last line for a key wins.  Path: nc_set_tune_cache, else the environment
+// last line for a key wins.  v3 lines come from timing on random operands;
+// v2 lines (timed on zeros) no longer match any key.  Path: nc_set_tune_cache, else the environment
 // variable NC_TUNE_CACHE, else no cache.
 std::mutex g_cache_mu;
 std::string g_cache_path;
@@ -248,7 +249,7 @@ std::string cache_key(const nc_plan* p, uint32_t batch) {
     for (char& ch : arch)
         if (ch == ' ' || ch == ':') ch = '_';
     char buf[256];
-    std::snprintf(buf, sizeof buf, "nc-tune-v2 %s %d %u %u %u %u %u %u %d %u %d", arch.c_str(),
+    std::snprintf(buf, sizeof buf, "nc-tune-v3 %s %d %u %u %u %u %u %u %d %u %d", arch.c_str(),
                   prop.multiProcessorCount, p->host.N, p->host.L, p->host.q_bits, k_of(p, 62), k_of(p, 30),
                   bucket_of(batch), p->mfma_mode, p->chunks_forced, p->persist_mode);
     return buf;
