@@ -1731,7 +1731,18 @@ __device__ __forceinline__ bool persist_wait(uint32_t* cnt, uint32_t need, uint3
 }
 
 template <int LOGB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void persist30_kernel(const PersistArgs A) {
+// Two waves per SIMD (two workgroups per compute unit, a 256-VGPR budget).
+// Built for three, the kernel had 168 VGPRs and spilled about 104 VGPRs and
+// 64 SGPRs (400-424 bytes of scratch per lane) across all three item paths,
+// and 2-7 % of its launches returned wrong products: one wave's share of a
+// D, T or C item's output wrong at a time (decomposition items included,
+// which read nothing but the operands and their LDS tables), with no
+// readiness-wait error, at three and at two workgroups per compute unit, with
+// per-wave agent-scope acquires, and with agent-scope releases too.  At this
+// budget it spills 16 VGPRs once at entry (68 bytes) and returned no wrong
+// product in 1200 launches with the same schedule (N = 2^14 at batch 128 and
+// 256, N = 2^12 at batch 64, two operand sets alternating).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void persist30_kernel(const PersistArgs A) {
     using S = InnerShape<LOGB>;
     static_assert(S::WG == 256 && LOGB <= 12, "one 256-thread workgroup per item, blocks of at most 2^12");
     using CL = CrtLds<4, kPersistCrtPitch, 28, 32, 2 * 8 * 8>;
@@ -2202,15 +2213,9 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
     return hipGetLastError();
 }
 
-// The persistent form is switched off: its launches returned wrong products
-// now and then (three of 128 in one of three repeated launches of a
-// 128-product batch; one of three 256-product launches differing from the
-// launch sequence), with no readiness-wait error, and again after the CRT
-// items' L1 invalidate was corrected to agent scope.  Reading the hand-off
-// protocol found no further cause, so the kernel stays in the source and its
-// schedule stays checked on the CPU (persist_emulate), but nothing launches
-// it: every call runs the launch sequence.
-constexpr bool kPersistLaunchEnabled = false;
+// The persistent form is launched (set false to route every call through
+// the launch sequence).
+constexpr bool kPersistLaunchEnabled = true;
 
 bool persist_launch_enabled() { return kPersistLaunchEnabled; }
 
@@ -2255,10 +2260,10 @@ hipError_t launch_persist30(const DevPlan& d, const uint32_t* a, const uint32_t*
     persist_order(sh, A.order);
     const uint32_t words = d.xcds * A.xcd_words;
     persist_reset_kernel<<<1, 256, 0, s>>>(ctl, words);
-    // three workgroups per compute unit (53 KB of LDS and 168 VGPRs each);
-    // workgroups go round-robin over the XCDs, so each XCD's queue gets
-    // 3 cus / xcds of them
-    const dim3 grid(3 * d.cus);
+    // two workgroups per compute unit (53 KB of LDS and up to 256 VGPRs
+    // each); workgroups go round-robin over the XCDs, so each XCD's queue gets
+    // 2 cus / xcds of them
+    const dim3 grid(2 * d.cus);
     A.lag = persist_lag(sh.R, grid.x / d.xcds);
     switch (d.log_n) {
         case 10: persist30_kernel<8><<<grid, 256, 0, s>>>(A); break;

@@ -113,7 +113,7 @@ hipError_t launch_crt_inv2(const DevPlan& d, const uint32_t* res, uint32_t* out,
 // matrix-core paths at outer split 2, L = 28, 57..64 primes, N = 2^10..2^14,
 // a known XCD count (at most kPersistMaxXcd) and batch >= that count.
 bool persist_available(const DevPlan& d, uint32_t batch);
-bool persist_launch_enabled();  // false: the form is switched off in this build
+bool persist_launch_enabled();  // false: every call runs the launch sequence
 // Control block bytes of one launch (placed by the caller in device memory
 // that no other launch uses meanwhile).
 size_t persist_ctl_bytes(const DevPlan& d, uint32_t batch);

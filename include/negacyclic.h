@@ -180,11 +180,9 @@ int nc_plan_set_mfma(nc_plan* plan, int mode);
  * tuned), 0 never, 1 wherever available.  wait_bound_us bounds one readiness
  * wait inside the launch (0: 1 s); a wait that reaches it stops its XCD's
  * queue, so the launch still ends, and sets the plan's error word
- * (nc_plan_persist_status).  Output bits never depend on the mode.
- * The form is switched off in this build (mode 1 returns
- * NC_ERR_UNSUPPORTED; -1 and 0 are accepted and run the launch sequence):
- * its launches returned wrong products now and then, for a cause not yet
- * found.
+ * (nc_plan_persist_status).  Output bits never depend on the mode.  A
+ * library built with the form switched off returns NC_ERR_UNSUPPORTED for
+ * mode 1 (-1 and 0 then run the launch sequence).
  * (No counterpart in the reference.) */
 int nc_plan_set_persist(nc_plan* plan, int mode, uint32_t wait_bound_us);
 /* *persistent = 1 if calls with `batch` products run the persistent form. */

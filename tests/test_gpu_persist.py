@@ -7,7 +7,8 @@ Sizes: every degree the form covers (N = 2^10 .. 2^14) at batch 8 (one
 product per XCD), uneven batches, the 128- and 256-product batches at
 N = 2^14 with every product checked, a second 868-bit modulus; edge inputs
 (all q - 1, zeros); the mixed-degree graph and the allocation-free entry
-point; the error path forced from the host with a 1 us readiness bound.
+point; the error path forced from the host with a 1 us readiness bound;
+hundreds of launches in a row against the launch sequence.
 """
 from __future__ import annotations
 
@@ -17,14 +18,7 @@ import pytest
 import ncmul
 import oracle as O
 
-# The library has the persistent form switched off (nc_plan_set_persist mode
-# 1 returns NC_ERR_UNSUPPORTED): its launches returned wrong products now and
-# then -- three of 128 in one of three repeated launches of the 128-product
-# batch, one of three 256-product bench launches -- for a cause not yet found.
-# These tests stay as they are, skipped, until the form is switched on again;
-# tests/test_gpu_persist_off.py checks the switch.
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="persistent form switched off: intermittent wrong products, cause not found")]
+pytestmark = pytest.mark.gpu
 
 L = 28
 
@@ -251,3 +245,49 @@ def test_persist_batch_256(gpu, q868):
         assert plan.persist_status() == 0
     bad = [i for i in range(B) if not np.array_equal(got[i], orc.multiply(a[i], b[i], threads=0))]
     assert not bad, bad[:8]
+
+
+def test_persist_repeated_launches(gpu, q868):
+    """Hundreds of launches in a row, two operand sets alternating and the
+    output cleared before each, every launch equal to the launch sequence's
+    products (two of which are checked against the oracle).  Built for three
+    waves per SIMD (168 VGPRs, about 104 VGPRs and 64 SGPRs spilled), the
+    kernel returned one wave's share of wrong output in 2-7 % of such
+    launches; the two-wave build (16 VGPRs spilled once at entry) returned
+    none in 1200."""
+    import torch
+
+    for N, B, launches in ((4096, 64, 300), (16384, 128, 150), (16384, 256, 60)):
+        say("persist repeated", N, B, launches)
+        sets = [batch_inputs(N, B, q868, 15000 + 1000 * s_) for s_ in range(2)]
+        dev = [(ncmul.to_device_u32(a), ncmul.to_device_u32(b)) for a, b in sets]
+        d_c = ncmul.empty_device_u32((B, N, L))
+        with ncmul.Plan(N, q868) as plan:
+            plan.set_tuning(30, 2)
+            plan.set_chunks(1)
+            plan.set_persist(0)
+            refs = []
+            for d_a, d_b in dev:
+                d_c.fill_(-1)
+                plan.multiply_device(d_a, d_b, d_c, B)
+                torch.cuda.synchronize()
+                refs.append(d_c.clone())
+            if N == 4096:
+                orc = O.Plan(N, q868)
+                got = ncmul.from_device_u32(refs[0])
+                for i in (0, B - 1):
+                    assert np.array_equal(got[i], orc.multiply(sets[0][0][i], sets[0][1][i], threads=0)), i
+            plan.set_persist(1)
+            if not plan.persistent(B):
+                pytest.skip("the device does not report its XCD count")
+            bad = []
+            for it in range(launches):
+                d_a, d_b = dev[it & 1]
+                d_c.fill_(-1)
+                plan.multiply_device(d_a, d_b, d_c, B)
+                torch.cuda.synchronize()
+                if not torch.equal(d_c, refs[it & 1]):
+                    wrong = torch.nonzero((d_c != refs[it & 1]).reshape(B, -1).any(dim=1)).flatten().tolist()
+                    bad.append((it, wrong[:8]))
+            assert plan.persist_status() == 0
+            assert not bad, (N, B, len(bad), bad[:4])
