@@ -160,7 +160,8 @@ def test_persist_in_mixed_degree_graph(gpu, q868):
     degree: one persistent launch per group of 2^10 .. 2^14 (groups below
     one product per XCD and 2^15 keep the launch sequence), captured in one
     HIP graph, three replays with eager work between them, every product
-    equal to the plain path's."""
+    equal to the plain path's; then 20 more replays, each compared on the
+    device with the first."""
     import torch
     from test_gpu_batch_serve import make_groups, mixed_degrees, plain_products
 
@@ -182,6 +183,15 @@ def test_persist_in_mixed_degree_graph(gpu, q868):
                 assert np.array_equal(ncmul.from_device_u32(c), want[N]), (rep, N)
             batch.run(groups[:2])
             torch.cuda.synchronize()
+        first = [c.clone() for _, _, _, _, c in groups]
+        for rep in range(20):
+            for _, _, _, _, c in groups:
+                c.fill_(-1)
+            graph.launch()
+            torch.cuda.synchronize()
+            for (N, _, _, _, c), f in zip(groups, first):
+                assert torch.equal(c, f), (rep, N)
+        assert batch.persist_status(all_n) == 0
 
 
 def test_persist_noalloc_scratch(gpu, q868):
